@@ -1,0 +1,341 @@
+"""Benchmark of the gfx950 BLAS hot path; prints ONE JSON line.
+
+Headline (`value`): split-BF16 SGEMM, M = N = K = 8192, row-major, operands
+U[-1,1) from seed 2, C = 0: FP32-equivalent TFLOP/s = 2MNK / step time, where
+one step is one omb_sgemm(mode = split-BF16) call (both split passes plus the
+MFMA kernel), operands already resident in HBM.
+
+Beside it: plain FP32 SGEMM 8192^3 (row- and column-major, seed 1), the
+error of every GEMM variant against an FP64 product (both input
+distributions), SGEMV 16384^2 and SDOT 2^28 in GB/s, the roofline of each
+kernel from HIP events on the launch stream, and the reference algorithm
+(the oracle, OpenMP on the host cores) timed on the same 8192^3 FP32 GEMM.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import __graft_entry__ as graft  # noqa: E402
+
+METRIC = ("SGEMM TFLOP/s 8192^3 split-BF16 vs plain FP32 @≤2x FP32 err; "
+          "GEMV GB/s; p99 ms")
+PEAK_HBM = 8.0e12
+PEAK_MFMA_BF16 = 2.5e15
+PEAK_FP32 = 157.3e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class Timer:
+    """HIP events recorded on the launch stream."""
+
+    def __init__(self, torch, stream):
+        self.torch, self.stream = torch, stream
+
+    def run(self, fn, iters):
+        t = self.torch
+        e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+        e0.record(self.stream)
+        for _ in range(iters):
+            fn()
+        e1.record(self.stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / iters / 1e3  # seconds per call
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-errors", action="store_true", help="skip the FP64 error legs")
+    ap.add_argument("--cpu-budget", type=float, default=25.0,
+                    help="seconds of host time for the CPU baseline sample")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is a design, not a measured path)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    if not graft.built():
+        graft.build()
+    pkg = graft.load_package()
+    lib = pkg.lib()
+    pkg.check(lib.omb_init())
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+    timer = Timer(torch, stream)
+    out: dict = {}
+
+    def sync():
+        torch.cuda.synchronize()
+
+    n = 8192
+    flop = 2.0 * n * n * n
+    with torch.cuda.stream(stream):
+        A = torch.empty((n, n), device=dev)
+        B = torch.empty((n, n), device=dev)
+        C = torch.zeros((n, n), device=dev)
+    pkg.fill_uniform(A, 2, 0, stream=sp)
+    pkg.fill_uniform(B, 2, 1, stream=sp)
+    sync()
+
+    def gemm(mode, a=A, b=B, c=C, beta=0.0, layout="row"):
+        if layout == "row":
+            return lib.omb_sgemm(n, n, n, 1.0, a.data_ptr(), n, 1, b.data_ptr(), n, 1, beta,
+                                 c.data_ptr(), n, 1, mode, sp)
+        return lib.omb_sgemm(n, n, n, 1.0, a.data_ptr(), 1, n, b.data_ptr(), 1, n, beta,
+                             c.data_ptr(), 1, n, mode, sp)
+
+    # ------------------------------------------------ headline: split-BF16 --
+    log("headline: split-BF16 sgemm 8192^3")
+    for _ in range(args.warmup):
+        pkg.check(gemm(pkg.SPLIT_BF16))
+    sync()
+    t0 = time.perf_counter()
+    t_step = timer.run(lambda: gemm(pkg.SPLIT_BF16), args.steps)
+    sync()
+    wall = time.perf_counter() - t0
+    value = flop / t_step / 1e12
+
+    # the MFMA kernel alone (pre-packed operands), for the roofline
+    pbytes = lib.omb_split_pack_bytes(n, n)
+    with torch.cuda.stream(stream):
+        Ap = torch.empty(pbytes, dtype=torch.uint8, device=dev)
+        Bp = torch.empty(pbytes, dtype=torch.uint8, device=dev)
+    pkg.check(lib.omb_split_pack(n, n, A.data_ptr(), n, 1, Ap.data_ptr(), sp))
+    pkg.check(lib.omb_split_pack(n, n, B.data_ptr(), 1, n, Bp.data_ptr(), sp))  # B^T view
+    t_pack = timer.run(lambda: lib.omb_split_pack(n, n, A.data_ptr(), n, 1, Ap.data_ptr(), sp),
+                       max(3, args.steps // 2))
+    kernels = {}
+    for nprod, name in ((6, "sgemm_bf16x3_kernel<6>"), (3, "sgemm_bf16x3_kernel<3>")):
+        fn = lambda: lib.omb_sgemm_packed(n, n, n, 1.0, Ap.data_ptr(), Bp.data_ptr(), 0.0,
+                                          C.data_ptr(), n, 1, nprod, sp)
+        for _ in range(2):
+            fn()
+        t_k = timer.run(fn, args.steps)
+        kernels[name] = {"ms": t_k * 1e3, "mfma_tflops": nprod * flop / t_k / 1e12,
+                         "fp32_equiv_tflops": flop / t_k / 1e12,
+                         "frac_of_bf16_peak": nprod * flop / t_k / PEAK_MFMA_BF16}
+    t_k6 = kernels["sgemm_bf16x3_kernel<6>"]["ms"] / 1e3
+    kernels["split_pack_kernel (8192x8192)"] = {
+        "ms": t_pack * 1e3, "gbps": (4 + 6) * n * n / t_pack / 1e9,
+        "frac_of_hbm_peak": (4 + 6) * n * n / t_pack / PEAK_HBM}
+    del Ap, Bp
+
+    # split 3-product step time
+    t_step3 = timer.run(lambda: gemm(pkg.SPLIT_BF16_P3), max(3, args.steps // 2))
+
+    # --------------------------------------------------------- plain FP32 --
+    log("plain fp32 sgemm 8192^3 (seed 1, row and column major)")
+    pkg.fill_uniform(A, 1, 0, stream=sp)
+    pkg.fill_uniform(B, 1, 1, stream=sp)
+    plain = {}
+    for layout in ("row", "col"):
+        fn = lambda: gemm(pkg.PLAIN_F32, layout=layout)
+        fn()
+        t_p = timer.run(fn, max(3, args.steps // 2))
+        plain[layout] = {"ms": t_p * 1e3, "tflops": flop / t_p / 1e12,
+                         "frac_of_fp32_peak": flop / t_p / PEAK_FP32}
+    kernels["sgemm_f32_kernel (8192^3 row-major)"] = dict(plain["row"])
+
+    # ------------------------------------------------------------- errors --
+    errors = {}
+    if not args.no_errors:
+        errors = error_leg(pkg, lib, torch, dev, sp, A, B, C)
+    sync()
+
+    # --------------------------------------------------------- GEMV / dot --
+    log("sgemv 16384^2 and sdot 2^28")
+    del C
+    m = 16384
+    with torch.cuda.stream(stream):
+        GA = torch.empty((m, m), device=dev)
+        gx = torch.empty(m, device=dev)
+        gy = torch.zeros(m, device=dev)
+    pkg.fill_uniform(GA, 1, 0, stream=sp)
+    pkg.fill_uniform(gx, 1, 1, stream=sp)
+    gemv = {}
+    for layout, (rs, cs, beta) in (("row", (m, 1, 0.0)), ("col", (1, m, 1.0))):
+        fn = lambda: lib.omb_sgemv(m, m, 1.0, GA.data_ptr(), rs, cs, gx.data_ptr(), beta,
+                                   gy.data_ptr(), sp)
+        for _ in range(3):
+            fn()
+        t_g = timer.run(fn, max(10, args.steps))
+        nbytes = 4.0 * (m * m + m + m) + (4.0 * m if beta != 0.0 else 0.0)
+        gemv[layout] = {"us": t_g * 1e6, "gbps": nbytes / t_g / 1e9,
+                        "frac_of_hbm_peak": nbytes / t_g / PEAK_HBM, "bytes": nbytes}
+    kernels["sgemv_row_kernel (16384^2)"] = gemv["row"]
+    kernels["sgemv_col_kernel (16384^2)"] = gemv["col"]
+    del GA
+    nd = 1 << 28
+    with torch.cuda.stream(stream):
+        dx = torch.empty(nd, device=dev)
+        dy = torch.empty(nd, device=dev)
+        dr = torch.zeros(1, device=dev)
+    pkg.fill_uniform(dx, 1, 0, stream=sp)
+    pkg.fill_uniform(dy, 1, 1, stream=sp)
+    fn = lambda: lib.omb_sdot(nd, dx.data_ptr(), dy.data_ptr(), dr.data_ptr(), sp)
+    for _ in range(3):
+        fn()
+    t_d = timer.run(fn, max(10, args.steps))
+    dot = {"us": t_d * 1e6, "gbps": 8.0 * nd / t_d / 1e9, "frac_of_hbm_peak": 8.0 * nd / t_d / PEAK_HBM}
+    kernels["sdot_kernel (2^28)"] = dot
+    del dx, dy
+    sync()
+
+    # ------------------------------------------------------- CPU baseline --
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args.cpu_budget)
+
+    res = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_step * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "arith": "FP32 operands split into 3 BF16 terms, 6 MFMA products, FP32 accumulation",
+        "data": "synthetic: counter-based U[-1,1) from seed 2 (operands resident in HBM)",
+        "config": {"workload": "sgemm split-bf16 M=N=K=8192 row-major, C = A*B",
+                   "M": n, "N": n, "K": n, "layout": "row-major", "seed": 2},
+        "roofline": {"bound": "mfma", "kernel": "sgemm_bf16x3_kernel<6>",
+                     "achieved": round(6 * flop / t_k6 / 1e12, 2), "peak": 2500.0,
+                     "unit": "TFLOP/s", "frac": round(6 * flop / t_k6 / PEAK_MFMA_BF16, 4),
+                     "traffic": None,
+                     "work_per_launch": "6 products x 2*8192^3 = 6.597e12 MFMA flop"},
+        "cpu_baseline": cpu,
+        "wall_s_timed_region": round(wall, 4),
+        "split_bf16_p3_step_ms": round(t_step3 * 1e3, 4),
+        "plain_f32": plain,
+        "speedup_split_vs_plain": round(plain["row"]["ms"] / (t_step * 1e3), 3),
+        "errors": errors,
+        "gemv": gemv,
+        "dot": dot,
+        "kernels": kernels,
+        "device": torch.cuda.get_device_name(0),
+    }
+    print(json.dumps(res), flush=True)
+
+
+def error_leg(pkg, lib, torch, dev, sp, A, B, C):
+    """Max- and Frobenius-relative error vs an FP64 product (GPU check kernel)
+    of plain FP32 and split-BF16 (6 and 3 products) at 8192^3, for U[-1,1)
+    and sign*2^u (u ~ U[-8,8]) inputs from seed 2.  The reference algorithm's
+    (oracle's) error is measured on a 64-row slab, whose values do not depend
+    on the other rows, and compared with the GPU's on the same slab."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import helpers
+
+    n = 8192
+    rows = slice(4096, 4160)
+    out = {}
+    C64 = torch.empty((n, n), dtype=torch.float64, device=dev)
+    for dist in ("uniform", "pow2"):
+        log(f"errors vs FP64: {dist}")
+        if dist == "uniform":
+            pkg.fill_uniform(A, 2, 0, stream=sp)
+            pkg.fill_uniform(B, 2, 1, stream=sp)
+            torch.cuda.synchronize()
+            a_h, b_h = A.cpu().numpy(), B.cpu().numpy()
+        else:
+            a_h = helpers.rng_pow2(n * n, 2, 0).reshape(n, n)
+            b_h = helpers.rng_pow2(n * n, 2, 1).reshape(n, n)
+            A.copy_(torch.from_numpy(a_h))
+            B.copy_(torch.from_numpy(b_h))
+            torch.cuda.synchronize()
+        pkg.check(lib.omb_check_dgemm(n, n, n, A.data_ptr(), n, 1, B.data_ptr(), n, 1, None, n, 1,
+                                      C64.data_ptr(), sp))
+        torch.cuda.synchronize()
+        truth = C64[rows].cpu().numpy()
+        c_ref = np.zeros((64, n), np.float32)
+        helpers.oracle_mtm(c_ref, np.ascontiguousarray(a_h[rows]), b_h)
+        ref = helpers.rel_errors(c_ref, truth)
+        d = {"oracle_slab": {"max_rel": ref[0], "frob_rel": ref[1]}}
+        for mode, name in ((pkg.PLAIN_F32, "plain_f32"), (pkg.SPLIT_BF16, "split_bf16_p6"),
+                           (pkg.SPLIT_BF16_P3, "split_bf16_p3")):
+            pkg.check(lib.omb_sgemm(n, n, n, 1.0, A.data_ptr(), n, 1, B.data_ptr(), n, 1, 0.0,
+                                    C.data_ptr(), n, 1, mode, sp))
+            torch.cuda.synchronize()
+            full = pkg.rel_errors(C, C64, stream=sp)
+            slab = helpers.rel_errors(C[rows].cpu().numpy(), truth)
+            d[name] = {"max_rel": full[0], "frob_rel": full[1],
+                       "slab_max_rel": slab[0], "slab_frob_rel": slab[1],
+                       "slab_ratio_vs_oracle": [slab[0] / ref[0], slab[1] / ref[1]]}
+        p = d["plain_f32"]
+        for name in ("split_bf16_p6", "split_bf16_p3"):
+            d[name]["ratio_vs_plain_f32"] = [d[name]["max_rel"] / p["max_rel"],
+                                             d[name]["frob_rel"] / p["frob_rel"]]
+        out[dist] = d
+    del C64
+    return out
+
+
+def cpu_baseline(budget_s: float):
+    """The oracle's mtm (the reference algorithm: BLIS 5-loop, MR=16, NR=6,
+    KC/MC/NC from this host's caches, OpenMP over all visible cores) on the
+    8192^3 FP32 product; mean of up to 4 timed calls after one warm-up
+    (include/benchmark.hpp:34-51, MaxIter = 4 for mtm)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import helpers
+
+    n = 8192
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    log(f"cpu baseline: oracle mtm 8192^3 on {cores} threads")
+    a = helpers.rng_uniform(n * n, 1, 0).reshape(n, n)
+    b = helpers.rng_uniform(n * n, 1, 1).reshape(n, n)
+    c = np.zeros((n, n), np.float32)
+    blk = helpers.host_blocking()
+    t0 = time.perf_counter()
+    helpers.oracle_mtm(c, a, b, blk, cores)
+    t_first = time.perf_counter() - t0
+    iters = int(max(1, min(4, (budget_s - t_first) // max(t_first, 1e-3))))
+    times = []
+    for _ in range(iters):
+        c[:] = 0
+        t0 = time.perf_counter()
+        helpers.oracle_mtm(c, a, b, blk, cores)
+        times.append(time.perf_counter() - t0)
+    mean = float(np.mean(times))
+    flop = 2.0 * n * n * n
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": round(flop / mean / 1e12, 4), "unit": "TFLOP/s", "cores": cores,
+            "kind": "port",
+            "sample": f"full 8192^3 FP32 mtm (C += A*B), mean of {iters} timed calls after "
+                      f"1 warm-up ({t_first:.2f} s)",
+            "median_s": float(np.median(times)), "mean_s": mean, "cpu_model": model,
+            "blocking": blk.as_dict(), "machine": platform.machine()}
+
+
+if __name__ == "__main__":
+    main()

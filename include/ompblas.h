@@ -1,0 +1,227 @@
+/*
+ * ompblas.h -- C ABI of the MI355X (gfx950) BLAS hot path: mtm (SGEMM),
+ * mtv / vtm (SGEMV) and dot (SDOT), plus the split-BF16 GEMM mode, the
+ * grouped persistent launch and the request-serving loop.
+ *
+ * Conventions
+ *   - Every matrix is described as in the reference's uBLAS helpers: extents
+ *     {rows, cols} and strides {rs, cs}, element (i, j) at p[i*rs + j*cs].
+ *     One of rs / cs must be 1 (first_order: rs == 1, last_order: cs == 1)
+ *     and the other at least the matching extent.
+ *   - All data pointers are DEVICE pointers unless a name says otherwise.
+ *   - Calls are asynchronous on `stream` (a hipStream_t; NULL = default
+ *     stream).  Calls on different streams may run concurrently: each
+ *     stream owns its library workspace.
+ *   - No exception crosses this boundary.  Status codes are <= 0;
+ *     omb_last_error() gives the calling thread's last message.
+ */
+#ifndef OMPBLAS_H
+#define OMPBLAS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t omb_status;
+#define OMB_OK 0
+#define OMB_EINVAL (-1)       /* bad argument (null pointer, negative size, ld < extent) */
+#define OMB_EUNSUPPORTED (-2) /* valid request this library does not implement */
+#define OMB_ENOMEM (-3)       /* device memory could not be obtained */
+#define OMB_EDEVICE (-4)      /* a HIP runtime call failed */
+#define OMB_EBUSY (-5)        /* admission refused (serving loop) */
+
+typedef void* omb_stream_t; /* hipStream_t */
+
+/* GEMM arithmetic modes. */
+#define OMB_MODE_PLAIN_F32 0     /* FP32 FMA on the vector ALUs (no MFMA) */
+#define OMB_MODE_SPLIT_BF16 1    /* FP32 operands split into 3 BF16 terms, 6 products on MFMA */
+#define OMB_MODE_SPLIT_BF16_P3 2 /* same split, 3 leading products only (faster, less accurate) */
+
+/* Last error message of the calling thread ("" when none). */
+const char* omb_last_error(void);
+
+/* Initialise the library on the current device (optional; idempotent). */
+omb_status omb_init(void);
+
+/* Library version string. */
+const char* omb_version(void);
+
+/* ---------------------------------------------------------------- mtm ----
+ * C = alpha*A*B + beta*C;  A: M x K, B: K x N, C: M x N.
+ * Replaces amt::mtm_helper (include/mtm.hpp:116-122), which computes
+ * C += A*B (alpha = beta = 1) for any first/last-order mix of A, B, C.
+ * beta == 0 does not read C. */
+omb_status omb_sgemm(int64_t M, int64_t N, int64_t K, float alpha,
+                     const float* A, int64_t rsA, int64_t csA,
+                     const float* B, int64_t rsB, int64_t csB,
+                     float beta, float* C, int64_t rsC, int64_t csC,
+                     int32_t mode, omb_stream_t stream);
+
+/* Split-BF16 in two steps, for callers that reuse an operand across calls.
+ * omb_split_pack splits an R x K FP32 operand (element (r,k) at
+ * X[r*rs + k*cs]) into the packed 3-plane BF16 form; pack A as (M x K) and
+ * B as its transpose (N x K: rs = csB, cs = rsB).  omb_sgemm_packed then
+ * computes C = alpha*A*B + beta*C from the two packed operands with
+ * nprod = 6 or 3 products.  `packed` needs omb_split_pack_bytes(R, K) bytes.
+ * omb_sgemm(mode = OMB_MODE_SPLIT_BF16*) is exactly these three calls. */
+size_t omb_split_pack_bytes(int64_t R, int64_t K);
+omb_status omb_split_pack(int64_t R, int64_t K, const float* X, int64_t rs, int64_t cs,
+                          void* packed, omb_stream_t stream);
+omb_status omb_sgemm_packed(int64_t M, int64_t N, int64_t K, float alpha,
+                            const void* A_packed, const void* Bt_packed, float beta,
+                            float* C, int64_t rsC, int64_t csC, int32_t nprod,
+                            omb_stream_t stream);
+
+/* ---------------------------------------------------------------- mtv ----
+ * y = alpha*A*x + beta*y;  A: M x N, x: N, y: M (contiguous vectors).
+ * Replaces amt::mtv_helper first_order (include/mtv.hpp:15-22; y += A*x,
+ * i.e. beta = 1) and last_order (include/mtv.hpp:72-79; y = A*x, beta = 0).
+ * vtm (include/mtv.hpp:164-236) is this call on the transposed view
+ * (swap M/N and rs/cs). */
+omb_status omb_sgemv(int64_t M, int64_t N, float alpha,
+                     const float* A, int64_t rsA, int64_t csA,
+                     const float* x, float beta, float* y, omb_stream_t stream);
+
+/* ---------------------------------------------------------------- dot ----
+ * *result = sum_i x[i]*y[i] (overwrite; result is a device pointer).
+ * Replaces amt::dot_prod_helper (include/dot.hpp:143-149).  Per-workgroup
+ * sums are FP32; the cross-workgroup combine is FP64 when n >= 2^26. */
+omb_status omb_sdot(int64_t n, const float* x, const float* y, float* result,
+                     omb_stream_t stream);
+
+/* ------------------------------------------------------- input filling ----
+ * Counter-based uniform[-1,1) generator: element i of (seed, stream_id) is
+ * top24(splitmix64(key(seed, stream_id), offset + i)) * 2^-23 - 1 (exact). */
+omb_status omb_fill_uniform(float* dst, int64_t n, uint64_t seed, uint64_t stream_id,
+                            int64_t offset, omb_stream_t stream);
+
+/* ---------------------------------------------------- FP64 error checks ----
+ * Measurement kernels (not part of the FP32 path): an FP64 GEMM / GEMV / dot
+ * of FP32 inputs and an error norm.  C64 is row-major M x N (ld = N).
+ * c0 (nullable, strides of C) is added first, so C64 = c0 + A*B. */
+omb_status omb_check_dgemm(int64_t M, int64_t N, int64_t K,
+                           const float* A, int64_t rsA, int64_t csA,
+                           const float* B, int64_t rsB, int64_t csB,
+                           const float* c0, int64_t rsC, int64_t csC,
+                           double* C64, omb_stream_t stream);
+omb_status omb_check_dgemv(int64_t M, int64_t N, const float* A, int64_t rsA, int64_t csA,
+                           const float* x, const float* y0, double* y64, omb_stream_t stream);
+omb_status omb_check_ddot(int64_t n, const float* x, const float* y, double* r64,
+                          omb_stream_t stream);
+/* out[0] = max|C - C64|, out[1] = max|C64|, out[2] = sum (C - C64)^2,
+ * out[3] = sum C64^2 (device array of 4 doubles).  Deterministic. */
+omb_status omb_check_errnorm(int64_t M, int64_t N, const float* C, int64_t rsC, int64_t csC,
+                             const double* C64, double* out, omb_stream_t stream);
+
+/* ----------------------------------------------------- grouped launch ----
+ * Thousands of independent GEMMs (C_i = alpha*A_i*B_i + beta*C_i, every
+ * operand row-major with the given leading dimensions) and GEMVs
+ * (y_j = alpha*A_j*x_j + beta*y_j, A_j row-major) executed by ONE
+ * persistent kernel launch, captured in a hipGraph.  No counterpart in the
+ * reference (it has no batched API); the per-instance semantics are those
+ * of omb_sgemm / omb_sgemv. */
+typedef struct {
+    int64_t M, N, K;
+    const float* A; int64_t lda;
+    const float* B; int64_t ldb;
+    float* C; int64_t ldc;
+    float alpha, beta;
+} omb_gemm_desc;
+
+typedef struct {
+    int64_t M, N;
+    const float* A; int64_t lda;
+    const float* x;
+    float* y;
+    float alpha, beta;
+} omb_gemv_desc;
+
+typedef struct omb_group_plan omb_group_plan;
+
+/* Build a plan: tile shapes from the autotuner table (see
+ * omb_autotune_group), work list on the device, graph captured on `stream`. */
+omb_status omb_group_plan_create(const omb_gemm_desc* gemms, int32_t n_gemm,
+                                 const omb_gemv_desc* gemvs, int32_t n_gemv,
+                                 int32_t mode, omb_stream_t stream, omb_group_plan** plan);
+/* Replay the captured graph (one persistent kernel, plus split pre-passes in
+ * split-BF16 mode). */
+omb_status omb_group_plan_launch(omb_group_plan* plan, omb_stream_t stream);
+/* Run the same work without the graph (direct kernel launches). */
+omb_status omb_group_plan_launch_direct(omb_group_plan* plan, omb_stream_t stream);
+/* Describe the plan: number of work items, tiles per class, the chosen
+ * tile class per shape bucket ("gemm:<cls> ..."), into buf (NUL-terminated). */
+omb_status omb_group_plan_describe(const omb_group_plan* plan, char* buf, size_t len);
+omb_status omb_group_plan_destroy(omb_group_plan* plan);
+
+/* Autotuner: time every tile class on representative shapes of each
+ * (M, N, K) log2 bucket present in `gemms`, keep the fastest per bucket in
+ * the process-wide table, and optionally persist it to `table_path`
+ * (text, one line per bucket; NULL = do not write).  Loading an existing
+ * table: omb_autotune_load. */
+omb_status omb_autotune_group(const omb_gemm_desc* gemms, int32_t n_gemm, int32_t mode,
+                              const char* table_path, omb_stream_t stream);
+omb_status omb_autotune_load(const char* table_path);
+
+/* ------------------------------------------------------ serving loop ----
+ * Continuous batching of mixed GEMM / GEMV / dot requests over device
+ * resident operands.  Requests are replayed from a trace of arrival times;
+ * each request is admitted when its HBM need fits the budget and a slot
+ * (stream) is free, runs concurrently with requests already on the GPU, and
+ * its completion time is recorded.  No counterpart in the reference. */
+#define OMB_REQ_GEMM 0
+#define OMB_REQ_GEMV 1
+#define OMB_REQ_DOT 2
+
+typedef struct {
+    int32_t kind;       /* OMB_REQ_* */
+    int32_t mode;       /* GEMM arithmetic mode (OMB_MODE_*) */
+    int64_t m, n, k;    /* GEMM: M,N,K; GEMV: rows=m, cols=n; dot: n */
+    double arrival_s;   /* arrival time, seconds from the start of the run */
+    const float* a;     /* device operands (row-major, contiguous) */
+    const float* b;     /* GEMM: B (K x N); GEMV: x; dot: y */
+    float* c;           /* GEMM: C (M x N); GEMV: y; dot: result (1 float) */
+    int64_t ws_bytes;   /* HBM the request needs beyond its operands (filled in) */
+} omb_request;
+
+typedef struct {
+    int32_t slots;          /* max requests in flight (one HIP stream each) */
+    int64_t hbm_budget;     /* bytes of workspace the loop may hold at once (0 = free HBM - reserve) */
+    int64_t hbm_reserve;    /* bytes kept free when hbm_budget == 0 */
+    int32_t spin_poll;      /* 1 = busy-poll completions, 0 = yield between polls */
+} omb_serve_config;
+
+typedef struct {
+    double start_s;     /* when the request was launched (from run start) */
+    double done_s;      /* when its completion was observed */
+    double latency_s;   /* done_s - arrival_s */
+    int32_t status;     /* OMB_OK or an error code */
+    int32_t slot;       /* stream slot it ran on */
+} omb_request_result;
+
+typedef struct {
+    double wall_s;
+    double p50_s, p90_s, p99_s, max_s, mean_s;
+    double total_flop;      /* sum of 2MNK / 2MN / 2n */
+    int32_t max_in_flight;  /* most requests observed on the GPU at once */
+    int32_t admitted_late;  /* requests that waited for a slot or for memory */
+} omb_serve_stats;
+
+omb_status omb_serve_run(omb_request* reqs, int32_t n, const omb_serve_config* cfg,
+                         omb_request_result* results, omb_serve_stats* stats);
+
+/* ----------------------------------------------- host-pointer helpers ----
+ * Used by the C++ drop-in layer to accept the reference's host buffers. */
+int32_t omb_is_device_pointer(const void* p);
+omb_status omb_device_alloc(void** p, size_t bytes);
+omb_status omb_device_free(void* p);
+omb_status omb_memcpy_h2d(void* dst, const void* src, size_t bytes, omb_stream_t stream);
+omb_status omb_memcpy_d2h(void* dst, const void* src, size_t bytes, omb_stream_t stream);
+omb_status omb_stream_synchronize(omb_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OMPBLAS_H */

@@ -1,0 +1,329 @@
+"""Host-side Python mirror of the reference's tensor-level BLAS calls, over
+the gfx950 C-ABI library ``lib/libompblas.so`` (ctypes).
+
+The reference's callers write ``amt::mtm(c, a, b, nthreads)()`` --
+the call validates shapes and returns a closure that runs the product
+(include/mtm.hpp:208-267, include/mtv.hpp:100-236, include/dot.hpp:176-224).
+This module keeps those names, argument meanings and error behaviour
+(``RuntimeError`` on shape mismatch when the closure is created) for torch
+tensors that live on the GPU:
+
+    mtm(c, a, b)()        C += A*B            (any row/column-major mix)
+    mtv(c, a, b)()        column-major A: c += A*b ; row-major A: c = A*b
+    vtm(c, a, b)()        c = A^T b (row-major A: accumulate, col-major: overwrite)
+    dot_prod(c, a, b)()   c[0] = sum a*b
+
+A tensor's layout is read from its strides: row-major ("last_order") when
+the column stride is 1, column-major ("first_order") when the row stride is 1.
+Every call runs the HIP kernels; there is no CPU fallback -- loading fails
+loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+from typing import Optional
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libompblas.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "ompblas.h")
+
+OK = 0
+EINVAL = -1
+EUNSUPPORTED = -2
+ENOMEM = -3
+EDEVICE = -4
+EBUSY = -5
+
+PLAIN_F32 = 0
+SPLIT_BF16 = 1
+SPLIT_BF16_P3 = 2
+
+REQ_GEMM, REQ_GEMV, REQ_DOT = 0, 1, 2
+
+_lib = None
+
+
+class OmbError(RuntimeError):
+    """A non-OK status from the C-ABI, with the library's message."""
+
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"ompblas status {status}: {msg}")
+        self.status = status
+
+
+class GemmDesc(ctypes.Structure):
+    _fields_ = [("M", ctypes.c_int64), ("N", ctypes.c_int64), ("K", ctypes.c_int64),
+                ("A", ctypes.c_void_p), ("lda", ctypes.c_int64),
+                ("B", ctypes.c_void_p), ("ldb", ctypes.c_int64),
+                ("C", ctypes.c_void_p), ("ldc", ctypes.c_int64),
+                ("alpha", ctypes.c_float), ("beta", ctypes.c_float)]
+
+
+class GemvDesc(ctypes.Structure):
+    _fields_ = [("M", ctypes.c_int64), ("N", ctypes.c_int64),
+                ("A", ctypes.c_void_p), ("lda", ctypes.c_int64),
+                ("x", ctypes.c_void_p), ("y", ctypes.c_void_p),
+                ("alpha", ctypes.c_float), ("beta", ctypes.c_float)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("mode", ctypes.c_int32),
+                ("m", ctypes.c_int64), ("n", ctypes.c_int64), ("k", ctypes.c_int64),
+                ("arrival_s", ctypes.c_double),
+                ("a", ctypes.c_void_p), ("b", ctypes.c_void_p), ("c", ctypes.c_void_p),
+                ("ws_bytes", ctypes.c_int64)]
+
+
+class ServeConfig(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_int32), ("hbm_budget", ctypes.c_int64),
+                ("hbm_reserve", ctypes.c_int64), ("spin_poll", ctypes.c_int32)]
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [("start_s", ctypes.c_double), ("done_s", ctypes.c_double),
+                ("latency_s", ctypes.c_double), ("status", ctypes.c_int32),
+                ("slot", ctypes.c_int32)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("wall_s", ctypes.c_double), ("p50_s", ctypes.c_double),
+                ("p90_s", ctypes.c_double), ("p99_s", ctypes.c_double),
+                ("max_s", ctypes.c_double), ("mean_s", ctypes.c_double),
+                ("total_flop", ctypes.c_double), ("max_in_flight", ctypes.c_int32),
+                ("admitted_late", ctypes.c_int32)]
+
+
+_I64, _F32, _VP, _I32, _U64, _F64 = (ctypes.c_int64, ctypes.c_float, ctypes.c_void_p,
+                                     ctypes.c_int32, ctypes.c_uint64, ctypes.c_double)
+
+_SIGS = {
+    "omb_last_error": (ctypes.c_char_p, []),
+    "omb_version": (ctypes.c_char_p, []),
+    "omb_init": (_I32, []),
+    "omb_sgemm": (_I32, [_I64, _I64, _I64, _F32, _VP, _I64, _I64, _VP, _I64, _I64, _F32, _VP,
+                         _I64, _I64, _I32, _VP]),
+    "omb_split_pack_bytes": (ctypes.c_size_t, [_I64, _I64]),
+    "omb_split_pack": (_I32, [_I64, _I64, _VP, _I64, _I64, _VP, _VP]),
+    "omb_sgemm_packed": (_I32, [_I64, _I64, _I64, _F32, _VP, _VP, _F32, _VP, _I64, _I64, _I32,
+                                _VP]),
+    "omb_sgemv": (_I32, [_I64, _I64, _F32, _VP, _I64, _I64, _VP, _F32, _VP, _VP]),
+    "omb_sdot": (_I32, [_I64, _VP, _VP, _VP, _VP]),
+    "omb_fill_uniform": (_I32, [_VP, _I64, _U64, _U64, _I64, _VP]),
+    "omb_check_dgemm": (_I32, [_I64, _I64, _I64, _VP, _I64, _I64, _VP, _I64, _I64, _VP, _I64,
+                               _I64, _VP, _VP]),
+    "omb_check_dgemv": (_I32, [_I64, _I64, _VP, _I64, _I64, _VP, _VP, _VP, _VP]),
+    "omb_check_ddot": (_I32, [_I64, _VP, _VP, _VP, _VP]),
+    "omb_check_errnorm": (_I32, [_I64, _I64, _VP, _I64, _I64, _VP, _VP, _VP]),
+    "omb_group_plan_create": (_I32, [ctypes.POINTER(GemmDesc), _I32, ctypes.POINTER(GemvDesc),
+                                     _I32, _I32, _VP, ctypes.POINTER(_VP)]),
+    "omb_group_plan_launch": (_I32, [_VP, _VP]),
+    "omb_group_plan_launch_direct": (_I32, [_VP, _VP]),
+    "omb_group_plan_describe": (_I32, [_VP, ctypes.c_char_p, ctypes.c_size_t]),
+    "omb_group_plan_destroy": (_I32, [_VP]),
+    "omb_autotune_group": (_I32, [ctypes.POINTER(GemmDesc), _I32, _I32, ctypes.c_char_p, _VP]),
+    "omb_autotune_load": (_I32, [ctypes.c_char_p]),
+    "omb_serve_run": (_I32, [ctypes.POINTER(Request), _I32, ctypes.POINTER(ServeConfig),
+                             ctypes.POINTER(RequestResult), ctypes.POINTER(ServeStats)]),
+    "omb_is_device_pointer": (_I32, [_VP]),
+    "omb_device_alloc": (_I32, [ctypes.POINTER(_VP), ctypes.c_size_t]),
+    "omb_device_free": (_I32, [_VP]),
+    "omb_memcpy_h2d": (_I32, [_VP, _VP, ctypes.c_size_t, _VP]),
+    "omb_memcpy_d2h": (_I32, [_VP, _VP, ctypes.c_size_t, _VP]),
+    "omb_stream_synchronize": (_I32, [_VP]),
+}
+
+
+def header_functions(path: str = HEADER_PATH) -> list:
+    """Names of every function the public header declares."""
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(omb_[a-z0-9_]+)\s*\(", text)))
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded C-ABI library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with "
+                               "`python -c 'import __graft_entry__ as g; g.build()'`")
+        l = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(l, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = l
+    return _lib
+
+
+def last_error() -> str:
+    return lib().omb_last_error().decode()
+
+
+def check(status: int) -> None:
+    if status != OK:
+        raise OmbError(status, last_error())
+
+
+def _stream(stream) -> Optional[int]:
+    if stream is None:
+        return None
+    return getattr(stream, "cuda_stream", stream)
+
+
+# --------------------------------------------------------------- raw ABI --
+def sgemm(M, N, K, alpha, A, rsA, csA, B, rsB, csB, beta, C, rsC, csC, mode=PLAIN_F32,
+          stream=None) -> int:
+    """Status of omb_sgemm on raw device pointers (ints)."""
+    return lib().omb_sgemm(M, N, K, alpha, A, rsA, csA, B, rsB, csB, beta, C, rsC, csC, mode,
+                           _stream(stream))
+
+
+def sgemv(M, N, alpha, A, rsA, csA, x, beta, y, stream=None) -> int:
+    return lib().omb_sgemv(M, N, alpha, A, rsA, csA, x, beta, y, _stream(stream))
+
+
+def sdot(n, x, y, result, stream=None) -> int:
+    return lib().omb_sdot(n, x, y, result, _stream(stream))
+
+
+# ------------------------------------------------------ torch conveniences --
+def _ptr(t) -> int:
+    return t.data_ptr()
+
+
+def _strides(t):
+    if t.dim() != 2:
+        raise RuntimeError("expected a matrix")
+    return t.stride(0), t.stride(1)
+
+
+def fill_uniform(t, seed: int, stream_id: int, offset: int = 0, stream=None) -> None:
+    """Fill a contiguous float32 device tensor with the counter-based U[-1,1)."""
+    check(lib().omb_fill_uniform(_ptr(t), t.numel(), seed, stream_id, offset, _stream(stream)))
+
+
+def is_matrix(t) -> bool:
+    return t.dim() == 2
+
+
+def is_vector(t) -> bool:
+    return t.dim() == 1 or (t.dim() == 2 and (t.shape[0] == 1 or t.shape[1] == 1))
+
+
+def mtm(c, a, b, num_threads=None, mode: int = PLAIN_F32, stream=None):
+    """amt::mtm (include/mtm.hpp:208-267): validate, return a closure computing C += A*B.
+
+    ``num_threads`` is accepted for signature parity and ignored (the GPU
+    decides its own parallelism)."""
+    if not (is_matrix(a) and is_matrix(b) and is_matrix(c)):
+        raise RuntimeError("amt::mtm: a, b, and c must be the matrices")
+    if not (a.shape[0] == c.shape[0] and a.shape[1] == b.shape[0] and c.shape[1] == b.shape[1]):
+        raise RuntimeError("amt::mtm: dimension mismatch")
+    M, K = a.shape
+    N = b.shape[1]
+    (rsa, csa), (rsb, csb), (rsc, csc) = _strides(a), _strides(b), _strides(c)
+
+    def run():
+        check(sgemm(M, N, K, 1.0, _ptr(a), rsa, csa, _ptr(b), rsb, csb, 1.0, _ptr(c), rsc, csc,
+                    mode, stream))
+
+    return run
+
+
+def _vec_len(t) -> int:
+    return t.numel()
+
+
+def _is_first_order(a) -> bool:
+    """Column-major (uBLAS first_order) when the row stride is 1."""
+    rs, cs = _strides(a)
+    if cs == 1 and rs == 1:
+        return a.shape[1] <= 1
+    return rs == 1
+
+
+def mtv(c, a, b, num_threads=None, stream=None):
+    """amt::mtv (include/mtv.hpp:100-162): c += A*b for column-major A
+    (first_order helper, :15-70), c = A*b for row-major A (last_order, :72-98)."""
+    if not (is_matrix(a) and is_vector(b) and is_vector(c)):
+        raise RuntimeError("amt::mtv: c and b must be vector, and a must be a matrix")
+    if a.shape[1] != _vec_len(b) or a.shape[0] != _vec_len(c):
+        raise RuntimeError("amt::mtv: dimension mismatch")
+    M, N = a.shape
+    rs, cs = _strides(a)
+    beta = 1.0 if _is_first_order(a) else 0.0
+
+    def run():
+        check(sgemv(M, N, 1.0, _ptr(a), rs, cs, _ptr(b), beta, _ptr(c), stream))
+
+    return run
+
+
+def vtm(c, a, b, num_threads=None, stream=None):
+    """amt::vtm (include/mtv.hpp:164-236): c = A^T b.  The reference runs the
+    helper of the other layout, so a column-major A overwrites c and a
+    row-major A accumulates into c; the product itself is the true A^T b."""
+    if not (is_matrix(a) and is_vector(b) and is_vector(c)):
+        raise RuntimeError("amt::vtm: c and b must be vector, and a must be a matrix")
+    if a.shape[1] != _vec_len(c) or a.shape[0] != _vec_len(b):
+        raise RuntimeError("amt::vtm: dimension mismatch")
+    M, N = a.shape
+    rs, cs = _strides(a)
+    beta = 0.0 if _is_first_order(a) else 1.0
+
+    def run():
+        # A^T is N x M with strides (cs, rs).
+        check(sgemv(N, M, 1.0, _ptr(a), cs, rs, _ptr(b), beta, _ptr(c), stream))
+
+    return run
+
+
+def dot_prod(c, a, b, num_threads=None, stream=None):
+    """amt::dot_prod (include/dot.hpp:176-224): c[0] = sum a*b (overwrite)."""
+    if not (is_vector(a) and is_vector(b)):
+        raise RuntimeError("amt::dot_prod: both tensor must be vector")
+    if _vec_len(a) != _vec_len(b):
+        raise RuntimeError("amt::dot_prod: both vector must be of same size")
+    n = _vec_len(a)
+
+    def run():
+        check(sdot(n, _ptr(a), _ptr(b), _ptr(c), stream))
+
+    return run
+
+
+# ------------------------------------------------------------ FP64 checks --
+def check_dgemm(a, b, c0=None, stream=None):
+    """FP64 product of FP32 device matrices (c0 + A*B) as a new float64 tensor."""
+    import torch
+    M, K = a.shape
+    N = b.shape[1]
+    out = torch.empty((M, N), dtype=torch.float64, device=a.device)
+    rsc, csc = _strides(c0) if c0 is not None else (N, 1)
+    check(lib().omb_check_dgemm(M, N, K, _ptr(a), *_strides(a), _ptr(b), *_strides(b),
+                                _ptr(c0) if c0 is not None else None, rsc, csc, _ptr(out),
+                                _stream(stream)))
+    return out
+
+
+def errnorm(c, c64, stream=None):
+    """(max|C-C64|, max|C64|, ||C-C64||_F^2, ||C64||_F^2) as Python floats."""
+    import torch
+    if c.dim() == 1:
+        c = c.view(-1, 1)
+        c64 = c64.view(-1, 1)
+    out = torch.empty(4, dtype=torch.float64, device=c.device)
+    M, N = c.shape
+    check(lib().omb_check_errnorm(M, N, _ptr(c), *_strides(c), _ptr(c64), _ptr(out),
+                                  _stream(stream)))
+    check(lib().omb_stream_synchronize(_stream(stream)))
+    return tuple(out.cpu().tolist())
+
+
+def rel_errors(c, c64, stream=None):
+    """(max-relative, Frobenius-relative) error of c against c64."""
+    dmax, rmax, d2, r2 = errnorm(c, c64, stream)
+    return dmax / rmax if rmax > 0 else dmax, (d2 / r2) ** 0.5 if r2 > 0 else d2 ** 0.5

@@ -1,0 +1,254 @@
+// FP64 measurement kernels: the high-precision product of the same FP32
+// inputs, and a deterministic error norm.  These run beside the FP32 path to
+// state its error; they are not part of any FP32 result.  Also the input
+// generator (uniform[-1,1) from a counter-based hash, bit-identical to the
+// host generator) and a small fill kernel.
+#include <algorithm>
+
+#include "common.hpp"
+
+namespace omb {
+namespace {
+
+constexpr int NT = 256;
+
+// 64x64 tile of C64, 16-wide k slices in LDS, 4x4 per thread; FP64 FMA.
+__global__ __launch_bounds__(NT) void dgemm_check_kernel(int64_t M, int64_t N, int64_t K,
+                                                         const float* A, int64_t rsA, int64_t csA,
+                                                         const float* B, int64_t rsB, int64_t csB,
+                                                         const float* c0, int64_t rsC, int64_t csC,
+                                                         double* C64, int64_t tiles_n) {
+    __shared__ double As[16][64 + 1];
+    __shared__ double Bs[16][64 + 1];
+    const int tid = threadIdx.x;
+    const int64_t tm = blockIdx.x / tiles_n, tn = blockIdx.x % tiles_n;
+    const int64_t m0 = tm * 64, n0 = tn * 64;
+    const int tx = tid & 15, ty = tid >> 4;
+    double acc[4][4] = {};
+    for (int64_t k0 = 0; k0 < K; k0 += 16) {
+        for (int e = tid; e < 64 * 16; e += NT) {
+            int r = e & 63, kk = e >> 6;
+            int64_t m = m0 + r, n = n0 + r, k = k0 + kk;
+            As[kk][r] = (m < M && k < K) ? (double)A[m * rsA + k * csA] : 0.0;
+            Bs[kk][r] = (n < N && k < K) ? (double)B[k * rsB + n * csB] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) {
+            double a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                a[i] = As[kk][ty + 16 * i];
+                b[i] = Bs[kk][tx + 16 * i];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            int64_t m = m0 + ty + 16 * i, n = n0 + tx + 16 * j;
+            if (m < M && n < N) {
+                double v = acc[i][j];
+                if (c0) v += (double)c0[m * rsC + n * csC];
+                C64[m * N + n] = v;
+            }
+        }
+}
+
+// One wavefront per row.
+__global__ __launch_bounds__(NT) void dgemv_check_kernel(int64_t M, int64_t N, const float* A,
+                                                         int64_t rsA, int64_t csA, const float* x,
+                                                         const float* y0, double* y64) {
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+    if (row >= M) return;
+    double s = 0.0;
+    for (int64_t j = lane; j < N; j += 64) s = fma((double)A[row * rsA + j * csA], (double)x[j], s);
+    s = wave_sum(s);
+    if (lane == 0) y64[row] = s + (y0 ? (double)y0[row] : 0.0);
+}
+
+__global__ __launch_bounds__(NT) void ddot_partial_kernel(int64_t n, const float* x, const float* y,
+                                                          double* part) {
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
+        s = fma((double)x[i], (double)y[i], s);
+    __shared__ double red[NT / 64];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < NT / 64; ++w) t += red[w];
+        part[blockIdx.x] = t;
+    }
+}
+
+// Sums `count` groups of `width` doubles in order (width 1: sum; the error
+// norm uses width 4 with max for slots 0-1 and sum for slots 2-3).
+__global__ void combine_kernel(const double* part, int64_t count, int width, double* out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t b = 0; b < count; ++b) {
+        for (int w = 0; w < width; ++w) {
+            double v = part[b * width + w];
+            if (width == 4 && w < 2) acc[w] = fmax(acc[w], v);
+            else acc[w] += v;
+        }
+    }
+    for (int w = 0; w < width; ++w) out[w] = acc[w];
+}
+
+__global__ __launch_bounds__(NT) void errnorm_partial_kernel(int64_t M, int64_t N, const float* C,
+                                                             int64_t rsC, int64_t csC,
+                                                             const double* C64, double* part) {
+    double dmax = 0.0, rmax = 0.0, d2 = 0.0, r2 = 0.0;
+    const int64_t total = M * N;
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * NT) {
+        int64_t m = i / N, n = i - m * N;
+        double ref = C64[i];
+        double d = (double)C[m * rsC + n * csC] - ref;
+        dmax = fmax(dmax, fabs(d));
+        rmax = fmax(rmax, fabs(ref));
+        d2 = fma(d, d, d2);
+        r2 = fma(ref, ref, r2);
+    }
+    dmax = wave_max(dmax);
+    rmax = wave_max(rmax);
+    d2 = wave_sum(d2);
+    r2 = wave_sum(r2);
+    __shared__ double red[NT / 64][4];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[w][0] = dmax;
+        red[w][1] = rmax;
+        red[w][2] = d2;
+        red[w][3] = r2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double o[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int i = 0; i < NT / 64; ++i) {
+            o[0] = fmax(o[0], red[i][0]);
+            o[1] = fmax(o[1], red[i][1]);
+            o[2] += red[i][2];
+            o[3] += red[i][3];
+        }
+        for (int i = 0; i < 4; ++i) part[blockIdx.x * 4 + i] = o[i];
+    }
+}
+
+__global__ __launch_bounds__(NT) void fill_uniform_kernel(float* dst, int64_t n, uint64_t key,
+                                                          int64_t offset) {
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
+        dst[i] = rng_uniform(key, (uint64_t)(i + offset));
+}
+
+__global__ __launch_bounds__(NT) void fill_u32_kernel(unsigned* dst, int64_t n, unsigned v) {
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
+        dst[i] = v;
+}
+
+unsigned grid_for(int64_t n, int64_t cap) {
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, NT), cap));
+}
+
+}  // namespace
+
+omb_status launch_fill_uniform(float* dst, int64_t n, uint64_t seed, uint64_t stream_id,
+                               int64_t offset, hipStream_t s) {
+    if (n <= 0) return OMB_OK;
+    hipLaunchKernelGGL(fill_uniform_kernel, dim3(grid_for(n, 16384)), dim3(NT), 0, s, dst, n,
+                       rng_key(seed, stream_id), offset);
+    OMB_HIP_TRY(hipGetLastError());
+    return OMB_OK;
+}
+
+omb_status launch_fill_u32(unsigned* dst, int64_t n, unsigned v, hipStream_t s) {
+    if (n <= 0) return OMB_OK;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(n, 4096)), dim3(NT), 0, s, dst, n, v);
+    OMB_HIP_TRY(hipGetLastError());
+    return OMB_OK;
+}
+
+}  // namespace omb
+
+using namespace omb;
+
+extern "C" omb_status omb_check_dgemm(int64_t M, int64_t N, int64_t K, const float* A,
+                                      int64_t rsA, int64_t csA, const float* B, int64_t rsB,
+                                      int64_t csB, const float* c0, int64_t rsC, int64_t csC,
+                                      double* C64, omb_stream_t stream) {
+    clear_error();
+    if (M < 0 || N < 0 || K < 0 || !C64 || (K > 0 && (!A || !B))) {
+        set_error("omb_check_dgemm: invalid argument");
+        return OMB_EINVAL;
+    }
+    if (M == 0 || N == 0) return OMB_OK;
+    const int64_t tiles_n = ceil_div(N, 64), tiles = ceil_div(M, 64) * tiles_n;
+    hipLaunchKernelGGL(dgemm_check_kernel, dim3((unsigned)tiles), dim3(NT), 0,
+                       (hipStream_t)stream, M, N, K, A, rsA, csA, B, rsB, csB, c0, rsC, csC, C64,
+                       tiles_n);
+    OMB_HIP_TRY(hipGetLastError());
+    return OMB_OK;
+}
+
+extern "C" omb_status omb_check_dgemv(int64_t M, int64_t N, const float* A, int64_t rsA,
+                                      int64_t csA, const float* x, const float* y0, double* y64,
+                                      omb_stream_t stream) {
+    clear_error();
+    if (M < 0 || N < 0 || !y64 || (N > 0 && (!A || !x))) {
+        set_error("omb_check_dgemv: invalid argument");
+        return OMB_EINVAL;
+    }
+    if (M == 0) return OMB_OK;
+    hipLaunchKernelGGL(dgemv_check_kernel, dim3((unsigned)ceil_div(M, NT / 64)), dim3(NT), 0,
+                       (hipStream_t)stream, M, N, A, rsA, csA, x, y0, y64);
+    OMB_HIP_TRY(hipGetLastError());
+    return OMB_OK;
+}
+
+extern "C" omb_status omb_check_ddot(int64_t n, const float* x, const float* y, double* r64,
+                                     omb_stream_t stream) {
+    clear_error();
+    if (n < 0 || !r64 || (n > 0 && (!x || !y))) {
+        set_error("omb_check_ddot: invalid argument");
+        return OMB_EINVAL;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const unsigned blocks = grid_for(n, 1024);
+    Workspace* ws = nullptr;
+    OMB_TRY(get_workspace(s, blocks * sizeof(double), &ws));
+    double* part = reinterpret_cast<double*>(ws->buf);
+    hipLaunchKernelGGL(ddot_partial_kernel, dim3(blocks), dim3(NT), 0, s, n, x, y, part);
+    hipLaunchKernelGGL(combine_kernel, dim3(1), dim3(1), 0, s, part, (int64_t)blocks, 1, r64);
+    OMB_HIP_TRY(hipGetLastError());
+    return OMB_OK;
+}
+
+extern "C" omb_status omb_check_errnorm(int64_t M, int64_t N, const float* C, int64_t rsC,
+                                        int64_t csC, const double* C64, double* out,
+                                        omb_stream_t stream) {
+    clear_error();
+    if (M < 0 || N < 0 || !out || (M * N > 0 && (!C || !C64))) {
+        set_error("omb_check_errnorm: invalid argument");
+        return OMB_EINVAL;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const unsigned blocks = grid_for(M * N, 1024);
+    Workspace* ws = nullptr;
+    OMB_TRY(get_workspace(s, (size_t)blocks * 4 * sizeof(double), &ws));
+    double* part = reinterpret_cast<double*>(ws->buf);
+    hipLaunchKernelGGL(errnorm_partial_kernel, dim3(blocks), dim3(NT), 0, s, M, N, C, rsC, csC,
+                       C64, part);
+    hipLaunchKernelGGL(combine_kernel, dim3(1), dim3(1), 0, s, part, (int64_t)blocks, 4, out);
+    OMB_HIP_TRY(hipGetLastError());
+    return OMB_OK;
+}

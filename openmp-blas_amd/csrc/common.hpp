@@ -1,0 +1,162 @@
+// Internal declarations shared by the kernels and the C-ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include <cstdint>
+#include <string>
+
+#include "ompblas.h"
+
+namespace omb {
+
+// ------------------------------------------------------------------ errors --
+void set_error(const std::string& msg);
+void clear_error();
+
+#define OMB_HIP_TRY(expr)                                                       \
+    do {                                                                        \
+        hipError_t e_ = (expr);                                                 \
+        if (e_ != hipSuccess) {                                                 \
+            ::omb::set_error(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+            return OMB_EDEVICE;                                                 \
+        }                                                                       \
+    } while (0)
+
+#define OMB_TRY(expr)                      \
+    do {                                   \
+        omb_status s_ = (expr);            \
+        if (s_ != OMB_OK) return s_;       \
+    } while (0)
+
+__host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+__host__ __device__ inline int64_t round_up(int64_t a, int64_t b) { return ceil_div(a, b) * b; }
+
+// --------------------------------------------------------------- workspace --
+// Per-stream scratch memory: one grow-only device buffer plus a block of
+// ticket counters that kernels leave at zero when they finish.
+struct Workspace {
+    void* buf = nullptr;
+    size_t bytes = 0;
+    unsigned* tickets = nullptr;  // kTickets counters, zero between calls
+    static constexpr int kTickets = 1 << 16;
+};
+
+// Workspace of `stream` on the current device with at least `bytes` of buffer.
+omb_status get_workspace(hipStream_t stream, size_t bytes, Workspace** ws);
+
+// ------------------------------------------------------- matrix operands --
+// A validated operand view: element (i, j) at p[i*rs + j*cs], one stride 1.
+struct MatView {
+    const float* p;
+    int64_t rows, cols, rs, cs;
+};
+
+// Resolve a view into (row_contig?, ld): row_contig means the column index
+// is the unit-stride one (cs == 1, "last_order"); ld is the other stride.
+omb_status resolve_layout(const char* name, int64_t rows, int64_t cols, int64_t rs, int64_t cs,
+                          bool* unit_col, int64_t* ld);
+
+// ------------------------------------------------------------ kernels -----
+// Plain FP32 SGEMM, row-major C (ldc), A K-contiguous iff a_kc, B N-contiguous
+// iff b_nc.  C = alpha*A*B + beta*C.
+omb_status launch_sgemm_f32(int64_t M, int64_t N, int64_t K, float alpha,
+                            const float* A, int64_t lda, bool a_kc,
+                            const float* B, int64_t ldb, bool b_nc,
+                            float beta, float* C, int64_t ldc, hipStream_t s);
+
+// Split-BF16 SGEMM (row-major C after layout folding); operands as strided views.
+omb_status launch_sgemm_split(int64_t M, int64_t N, int64_t K, float alpha,
+                              const float* A, int64_t rsA, int64_t csA,
+                              const float* B, int64_t rsB, int64_t csB,
+                              float beta, float* C, int64_t ldc, int nprod, hipStream_t s);
+
+// Packed split-BF16 operands (3 planes, 128-row x 16-k tiles, zero padded).
+int64_t split_packed_elems(int64_t R, int64_t K);
+omb_status launch_split_pack(int64_t R, int64_t K, const float* X, int64_t rs, int64_t cs,
+                             __bf16* out, hipStream_t s);
+omb_status launch_sgemm_packed(int64_t M, int64_t N, int64_t K, float alpha, const __bf16* ap,
+                               const __bf16* bp, float beta, float* C, int64_t ldc, int nprod,
+                               hipStream_t s);
+
+// Scale C (row-major) by beta (K == 0 case).
+omb_status launch_scale(int64_t M, int64_t N, float beta, float* C, int64_t ldc, hipStream_t s);
+
+omb_status launch_sgemv_row(int64_t M, int64_t N, float alpha, const float* A, int64_t lda,
+                            const float* x, float beta, float* y, hipStream_t s);
+omb_status launch_sgemv_col(int64_t M, int64_t N, float alpha, const float* A, int64_t lda,
+                            const float* x, float beta, float* y, hipStream_t s);
+omb_status launch_sdot(int64_t n, const float* x, const float* y, float* result, hipStream_t s);
+
+omb_status launch_fill_uniform(float* dst, int64_t n, uint64_t seed, uint64_t stream_id,
+                               int64_t offset, hipStream_t s);
+omb_status launch_fill_u32(unsigned* dst, int64_t n, unsigned v, hipStream_t s);
+
+// ----------------------------------------------------- device helpers -----
+#if defined(__HIPCC__)
+// Counter-based generator (splitmix64, Steele/Lea/Flood 2014).
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+__host__ __device__ inline uint64_t rng_key(uint64_t seed, uint64_t stream) {
+    return mix64(seed * 0x9E3779B97F4A7C15ULL + stream * 0xD1B54A32D192ED03ULL +
+                 0x632BE59BD9B4E019ULL);
+}
+__host__ __device__ inline float rng_uniform(uint64_t key, uint64_t i) {
+    uint64_t h = mix64(key + (i + 1) * 0x9E3779B97F4A7C15ULL);
+    return (float)(h >> 40) * 0x1.0p-23f - 1.0f;
+}
+
+// 128-bit streaming load that does not keep the line in cache.
+typedef float f32x4_nt __attribute__((ext_vector_type(4)));
+__device__ inline float4 nt_load4(const float4* p) {
+    f32x4_nt v = __builtin_nontemporal_load(reinterpret_cast<const f32x4_nt*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// Butterfly sum over the 64 lanes of a wavefront; every lane gets the total.
+// The pairing is fixed, so the result is deterministic.
+__device__ inline float wave_sum(float v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Map a 1-D block id to a tile id so that the blocks the hardware sends to
+// one XCD (ids congruent mod 8) get a contiguous range of tiles.
+__device__ inline int64_t xcd_remap(int64_t bid, int64_t nblocks) {
+    constexpr int64_t kXcd = 8;
+    int64_t full = (nblocks / kXcd) * kXcd;
+    if (bid >= full) return bid;
+    int64_t per = full / kXcd;
+    return (bid % kXcd) * per + bid / kXcd;
+}
+
+// Tile id -> (tm, tn) in groups of `group` tile rows (column-major inside a
+// group) so that tiles in flight together share A and B panels.
+__device__ inline void group_tile(int64_t t, int64_t tiles_m, int64_t tiles_n, int64_t group,
+                                  int64_t* tm, int64_t* tn) {
+    int64_t per_group = group * tiles_n;
+    int64_t g = t / per_group;
+    int64_t first = g * group;
+    int64_t gs = tiles_m - first < group ? tiles_m - first : group;
+    int64_t r = t - g * per_group;
+    *tm = first + r % gs;
+    *tn = r / gs;
+}
+#endif
+
+}  // namespace omb

@@ -1,0 +1,139 @@
+// SDOT for gfx950: *result = sum_i x[i]*y[i].
+//
+// Counterpart of dot_prod_helper (include/dot.hpp:143-174), whose three
+// regimes (serial simd, parallel-for reduction, blocked parallel reduction)
+// all reduce FP32 lane partials into per-thread and then per-team sums.
+// Here: a grid-stride loop of 128-bit loads (4 float4 of x and of y in flight
+// per lane) into FP32 lane sums, a fixed butterfly per wavefront, a fixed
+// LDS tree per workgroup, one partial per workgroup, and the last workgroup
+// (ticket counter) adds the partials in index order -- in FP64 when
+// n >= 2^26, in FP32 below.  The grid depends on n only, so the result is
+// bitwise reproducible.
+#include <algorithm>
+
+#include "common.hpp"
+
+namespace omb {
+namespace {
+
+constexpr int NT = 256;
+constexpr int UNROLL = 4;
+constexpr int kMaxBlocks = 2048;
+constexpr int64_t kF64Combine = int64_t(1) << 26;
+
+__device__ inline float dot4(float4 a, float4 b, float acc) {
+    acc = fmaf(a.x, b.x, acc);
+    acc = fmaf(a.y, b.y, acc);
+    acc = fmaf(a.z, b.z, acc);
+    acc = fmaf(a.w, b.w, acc);
+    return acc;
+}
+
+template <bool VEC>
+__global__ __launch_bounds__(NT) void sdot_kernel(int64_t n, const float* __restrict__ x,
+                                                  const float* __restrict__ y, float* result,
+                                                  double* partial, unsigned* ticket, int f64) {
+    const int64_t gtid = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t nthr = (int64_t)gridDim.x * NT;
+    float acc = 0.f;
+    int64_t tail = 0;
+    if (VEC) {
+        const int64_t n4 = n >> 2;
+        const float4* x4 = reinterpret_cast<const float4*>(x);
+        const float4* y4 = reinterpret_cast<const float4*>(y);
+        int64_t i = gtid;
+        for (; i + nthr * (UNROLL - 1) < n4; i += nthr * UNROLL) {
+            float4 a[UNROLL], b[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) {
+                a[u] = nt_load4(&x4[i + nthr * u]);
+                b[u] = nt_load4(&y4[i + nthr * u]);
+            }
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) acc = dot4(a[u], b[u], acc);
+        }
+        for (; i < n4; i += nthr) acc = dot4(x4[i], y4[i], acc);
+        tail = n4 << 2;
+    }
+    for (int64_t i = tail + gtid; i < n; i += nthr) acc = fmaf(x[i], y[i], acc);
+
+    // Workgroup sum: wavefront butterfly, then the 4 wave sums in order.
+    __shared__ float s_wave[NT / 64];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) s_wave[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (gridDim.x == 1) {
+        if (threadIdx.x == 0) {
+            float s = s_wave[0];
+            for (int w = 1; w < NT / 64; ++w) s += s_wave[w];
+            *result = s;
+        }
+        return;
+    }
+    if (threadIdx.x == 0) {
+        float s = s_wave[0];
+        for (int w = 1; w < NT / 64; ++w) s += s_wave[w];
+        partial[blockIdx.x] = (double)s;
+    }
+    __threadfence();
+    __syncthreads();
+    __shared__ unsigned s_last;
+    if (threadIdx.x == 0) s_last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    // Last workgroup: thread t adds partials t, t+NT, ... in order, then a
+    // fixed tree over the threads.
+    __shared__ double s_red[NT];
+    if (f64) {
+        double s = 0.0;
+        for (int64_t b = threadIdx.x; b < gridDim.x; b += NT) s += __builtin_nontemporal_load(&partial[b]);
+        s_red[threadIdx.x] = s;
+    } else {
+        float s = 0.f;
+        for (int64_t b = threadIdx.x; b < gridDim.x; b += NT) s += (float)__builtin_nontemporal_load(&partial[b]);
+        s_red[threadIdx.x] = (double)s;
+    }
+    __syncthreads();
+    for (int w = NT / 2; w >= 1; w >>= 1) {
+        if (threadIdx.x < w) {
+            if (f64) s_red[threadIdx.x] += s_red[threadIdx.x + w];
+            else s_red[threadIdx.x] = (double)((float)s_red[threadIdx.x] + (float)s_red[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *result = (float)s_red[0];
+        *ticket = 0u;
+    }
+}
+
+__global__ void zero_scalar_kernel(float* r) { *r = 0.f; }
+
+}  // namespace
+
+omb_status launch_sdot(int64_t n, const float* x, const float* y, float* result, hipStream_t s) {
+    if (n <= 0) {
+        hipLaunchKernelGGL(zero_scalar_kernel, dim3(1), dim3(1), 0, s, result);
+        OMB_HIP_TRY(hipGetLastError());
+        return OMB_OK;
+    }
+    // ~8 float4 of each operand per thread at least; up to 8 workgroups per CU.
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, ceil_div(n, NT * 4 * 8)));
+    Workspace* ws = nullptr;
+    OMB_TRY(get_workspace(s, (size_t)blocks * sizeof(double), &ws));
+    const bool vec = ((reinterpret_cast<uintptr_t>(x) & 15) == 0) &&
+                     ((reinterpret_cast<uintptr_t>(y) & 15) == 0);
+    const int f64 = n >= kF64Combine ? 1 : 0;
+    double* part = reinterpret_cast<double*>(ws->buf);
+    if (vec)
+        hipLaunchKernelGGL(sdot_kernel<true>, dim3((unsigned)blocks), dim3(NT), 0, s, n, x, y,
+                           result, part, ws->tickets, f64);
+    else
+        hipLaunchKernelGGL(sdot_kernel<false>, dim3((unsigned)blocks), dim3(NT), 0, s, n, x, y,
+                           result, part, ws->tickets, f64);
+    OMB_HIP_TRY(hipGetLastError());
+    return OMB_OK;
+}
+
+}  // namespace omb

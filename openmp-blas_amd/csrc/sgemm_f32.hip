@@ -1,0 +1,257 @@
+// Plain FP32 SGEMM for gfx950: C = alpha*A*B + beta*C on the vector ALUs.
+//
+// Counterpart of amt::mtm_helper / impl::mtm_kernel / simd_loop<MTM>
+// (include/mtm.hpp:83-206, include/simd_loop.hpp:80-192): the reference packs
+// MR x KC and KC x NR panels into cache-sized buffers and runs a 16x6 register
+// micro-kernel.  Here a 256-thread workgroup owns a 128x128 tile of C, stages
+// 128x16 slices of A and B through LDS (double buffered; the staging plays
+// the role of `pack`, including the layout change for either operand order),
+// and every thread keeps an 8x8 register micro-tile.
+//
+// Accumulation order: each thread sums 128 consecutive k (8 LDS slices) into
+// an inner register tile, then adds it to an outer tile.  This two-level sum
+// keeps the rounding error of long K at or below the reference's (which sums
+// KC consecutive k and then does C += block, simd_loop.hpp:120-128,160-190).
+//
+// Row-major C only: a column-major C is handled by the caller through
+// C^T = B^T A^T (swap operands and their strides).
+#include "common.hpp"
+
+namespace omb {
+namespace {
+
+constexpr int BM = 128, BN = 128, BK = 16;
+constexpr int NT = 256;         // threads per workgroup
+constexpr int INNER_K = 128;    // k steps per inner accumulation block
+constexpr int PAD = 4;          // LDS row padding (floats), keeps float4 alignment
+constexpr int kGroupM = 8;      // tile rows per L2 group
+
+struct Args {
+    int64_t M, N, K;
+    float alpha, beta;
+    const float* A;
+    int64_t lda;
+    const float* B;
+    int64_t ldb;
+    float* C;
+    int64_t ldc;
+    int64_t tiles_m, tiles_n;
+};
+
+// Load up to 4 consecutive floats starting at p (n valid, 0..4); zero-fill.
+template <bool VEC>
+__device__ inline float4 load4(const float* p, int n) {
+    if (VEC && n == 4) return *reinterpret_cast<const float4*>(p);
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (n > 0) v.x = p[0];
+    if (n > 1) v.y = p[1];
+    if (n > 2) v.z = p[2];
+    if (n > 3) v.w = p[3];
+    return v;
+}
+
+__device__ inline int clamp4(int64_t r) { return r <= 0 ? 0 : (r >= 4 ? 4 : (int)r); }
+
+// Global -> registers for one BK slice.  Each thread loads two float4 of A
+// and two of B along the operand's unit-stride dimension.
+template <bool A_KC, bool B_NC, bool VEC>
+__device__ inline void load_slice(const Args& p, int64_t m0, int64_t n0, int64_t k0, int tid,
+                                  float4 (&ra)[2], float4 (&rb)[2]) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        int e = tid + i * NT;
+        if (A_KC) {  // A[m*lda + k]: float4 along k
+            int row = e >> 2, kq = (e & 3) * 4;
+            int64_t m = m0 + row, k = k0 + kq;
+            int n = m < p.M ? clamp4(p.K - k) : 0;
+            ra[i] = load4<VEC>(p.A + (n ? m * p.lda + k : 0), n);
+        } else {     // A[k*lda + m]: float4 along m
+            int mq = (e & 31) * 4, kk = e >> 5;
+            int64_t m = m0 + mq, k = k0 + kk;
+            int n = k < p.K ? clamp4(p.M - m) : 0;
+            ra[i] = load4<VEC>(p.A + (n ? k * p.lda + m : 0), n);
+        }
+        if (B_NC) {  // B[k*ldb + n]: float4 along n
+            int nq = (e & 31) * 4, kk = e >> 5;
+            int64_t nn = n0 + nq, k = k0 + kk;
+            int n = k < p.K ? clamp4(p.N - nn) : 0;
+            rb[i] = load4<VEC>(p.B + (n ? k * p.ldb + nn : 0), n);
+        } else {     // B[n*ldb + k]: float4 along k
+            int col = e >> 2, kq = (e & 3) * 4;
+            int64_t nn = n0 + col, k = k0 + kq;
+            int n = nn < p.N ? clamp4(p.K - k) : 0;
+            rb[i] = load4<VEC>(p.B + (n ? nn * p.ldb + k : 0), n);
+        }
+    }
+}
+
+// Registers -> LDS, laid out [k][m] and [k][n].
+template <bool A_KC, bool B_NC>
+__device__ inline void store_slice(float (*As)[BM + PAD], float (*Bs)[BN + PAD], int tid,
+                                   const float4 (&ra)[2], const float4 (&rb)[2]) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        int e = tid + i * NT;
+        if (A_KC) {
+            int row = e >> 2, kq = (e & 3) * 4;
+            As[kq + 0][row] = ra[i].x;
+            As[kq + 1][row] = ra[i].y;
+            As[kq + 2][row] = ra[i].z;
+            As[kq + 3][row] = ra[i].w;
+        } else {
+            int mq = (e & 31) * 4, kk = e >> 5;
+            *reinterpret_cast<float4*>(&As[kk][mq]) = ra[i];
+        }
+        if (B_NC) {
+            int nq = (e & 31) * 4, kk = e >> 5;
+            *reinterpret_cast<float4*>(&Bs[kk][nq]) = rb[i];
+        } else {
+            int col = e >> 2, kq = (e & 3) * 4;
+            Bs[kq + 0][col] = rb[i].x;
+            Bs[kq + 1][col] = rb[i].y;
+            Bs[kq + 2][col] = rb[i].z;
+            Bs[kq + 3][col] = rb[i].w;
+        }
+    }
+}
+
+template <bool A_KC, bool B_NC, bool VEC>
+__global__ __launch_bounds__(NT, 2) void sgemm_f32_kernel(Args p) {
+    __shared__ float As[2][BK][BM + PAD];
+    __shared__ float Bs[2][BK][BN + PAD];
+
+    const int tid = threadIdx.x;
+    const int64_t ntiles = p.tiles_m * p.tiles_n;
+    int64_t t = xcd_remap(blockIdx.x, ntiles);
+    int64_t tm, tn;
+    group_tile(t, p.tiles_m, p.tiles_n, kGroupM, &tm, &tn);
+    const int64_t m0 = tm * BM, n0 = tn * BN;
+
+    const int tx = tid & 15, ty = tid >> 4;
+    float inner[8][8], outer[8][8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) inner[i][j] = outer[i][j] = 0.f;
+
+    const int64_t nk = ceil_div(p.K, BK);
+    float4 ra[2], rb[2];
+    load_slice<A_KC, B_NC, VEC>(p, m0, n0, 0, tid, ra, rb);
+    store_slice<A_KC, B_NC>(As[0], Bs[0], tid, ra, rb);
+    __syncthreads();
+
+    constexpr int kFold = INNER_K / BK;
+    for (int64_t kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        const bool more = kt + 1 < nk;
+        if (more) load_slice<A_KC, B_NC, VEC>(p, m0, n0, (kt + 1) * BK, tid, ra, rb);
+#pragma unroll
+        for (int k = 0; k < BK; ++k) {
+            float4 a0 = *reinterpret_cast<const float4*>(&As[cur][k][ty * 4]);
+            float4 a1 = *reinterpret_cast<const float4*>(&As[cur][k][64 + ty * 4]);
+            float4 b0 = *reinterpret_cast<const float4*>(&Bs[cur][k][tx * 4]);
+            float4 b1 = *reinterpret_cast<const float4*>(&Bs[cur][k][64 + tx * 4]);
+            const float a[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+            const float b[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) inner[i][j] = fmaf(a[i], b[j], inner[i][j]);
+        }
+        if ((kt + 1) % kFold == 0 || !more) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    outer[i][j] += inner[i][j];
+                    inner[i][j] = 0.f;
+                }
+        }
+        if (more) store_slice<A_KC, B_NC>(As[cur ^ 1], Bs[cur ^ 1], tid, ra, rb);
+        __syncthreads();
+    }
+
+    // Epilogue: rows {ty*4+i, 64+ty*4+i}, columns {tx*4+j, 64+tx*4+j}.
+    const bool cvec = ((reinterpret_cast<uintptr_t>(p.C) & 15) == 0) && (p.ldc % 4 == 0);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int64_t m = m0 + (i < 4 ? ty * 4 + i : 64 + ty * 4 + (i - 4));
+        if (m >= p.M) continue;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int64_t n = n0 + (h == 0 ? tx * 4 : 64 + tx * 4);
+            float* c = p.C + m * p.ldc + n;
+            float v[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[j] = p.alpha * outer[i][h * 4 + j];
+            if (cvec && n + 3 < p.N) {
+                float4 o;
+                if (p.beta != 0.f) {
+                    float4 c4 = *reinterpret_cast<const float4*>(c);
+                    o = make_float4(v[0] + p.beta * c4.x, v[1] + p.beta * c4.y,
+                                    v[2] + p.beta * c4.z, v[3] + p.beta * c4.w);
+                } else {
+                    o = make_float4(v[0], v[1], v[2], v[3]);
+                }
+                *reinterpret_cast<float4*>(c) = o;
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (n + j < p.N) c[j] = p.beta != 0.f ? v[j] + p.beta * c[j] : v[j];
+                }
+            }
+        }
+    }
+}
+
+template <bool A_KC, bool B_NC>
+void launch_variant(const Args& a, bool vec, hipStream_t s) {
+    dim3 grid((unsigned)(a.tiles_m * a.tiles_n)), block(NT);
+    if (vec)
+        hipLaunchKernelGGL((sgemm_f32_kernel<A_KC, B_NC, true>), grid, block, 0, s, a);
+    else
+        hipLaunchKernelGGL((sgemm_f32_kernel<A_KC, B_NC, false>), grid, block, 0, s, a);
+}
+
+__global__ void scale_kernel(int64_t M, int64_t N, float beta, float* C, int64_t ldc) {
+    int64_t total = M * N;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t m = i / N, n = i % N;
+        float* c = C + m * ldc + n;
+        *c = beta != 0.f ? beta * *c : 0.f;
+    }
+}
+
+}  // namespace
+
+omb_status launch_sgemm_f32(int64_t M, int64_t N, int64_t K, float alpha,
+                            const float* A, int64_t lda, bool a_kc,
+                            const float* B, int64_t ldb, bool b_nc,
+                            float beta, float* C, int64_t ldc, hipStream_t s) {
+    Args a{M, N, K, alpha, beta, A, lda, B, ldb, C, ldc, ceil_div(M, BM), ceil_div(N, BN)};
+    if (a.tiles_m * a.tiles_n > 0x7fffffffLL) {
+        set_error("sgemm: too many tiles for one launch");
+        return OMB_EUNSUPPORTED;
+    }
+    const bool vec = ((reinterpret_cast<uintptr_t>(A) & 15) == 0) &&
+                     ((reinterpret_cast<uintptr_t>(B) & 15) == 0) && (lda % 4 == 0) &&
+                     (ldb % 4 == 0);
+    if (a_kc && b_nc) launch_variant<true, true>(a, vec, s);
+    else if (a_kc && !b_nc) launch_variant<true, false>(a, vec, s);
+    else if (!a_kc && b_nc) launch_variant<false, true>(a, vec, s);
+    else launch_variant<false, false>(a, vec, s);
+    OMB_HIP_TRY(hipGetLastError());
+    return OMB_OK;
+}
+
+omb_status launch_scale(int64_t M, int64_t N, float beta, float* C, int64_t ldc, hipStream_t s) {
+    if (M <= 0 || N <= 0 || beta == 1.f) return OMB_OK;
+    int64_t total = M * N;
+    unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(total, 256), 4096);
+    hipLaunchKernelGGL(scale_kernel, dim3(blocks), dim3(256), 0, s, M, N, beta, C, ldc);
+    OMB_HIP_TRY(hipGetLastError());
+    return OMB_OK;
+}
+
+}  // namespace omb

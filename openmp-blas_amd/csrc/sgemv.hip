@@ -1,0 +1,245 @@
+// SGEMV for gfx950: y = alpha*A*x + beta*y, both storage orders of A.
+//
+// Row-major A (counterpart of mtv_helper last_order + simd_loop<INNER>,
+// include/mtv.hpp:72-98, include/simd_loop.hpp:21-35): one wavefront owns
+// ROWS consecutive rows; lanes stream 128-bit loads of each row, share every
+// x float4 across the ROWS rows, and finish with a butterfly lane sum.
+//
+// Column-major A (counterpart of mtv_helper first_order + simd_loop<MTV,16>,
+// include/mtv.hpp:15-70, include/simd_loop.hpp:52-78): every thread owns 4
+// consecutive rows (one 128-bit load per column); the columns are split into
+// chunks across workgroups, each chunk writes a partial y, and the last
+// workgroup of a row block (ticket counter) adds the partials in chunk order.
+// The result depends only on (M, N), never on scheduling.
+#include <algorithm>
+
+#include "common.hpp"
+
+namespace omb {
+namespace {
+
+constexpr int NT = 256;
+constexpr int ROWS = 4;    // rows per wavefront (row-major kernel)
+constexpr int UNROLL = 4;  // float4 loads in flight per row per lane
+
+template <bool VEC>
+__global__ __launch_bounds__(NT) void sgemv_row_kernel(int64_t M, int64_t N, float alpha,
+                                                       const float* __restrict__ A, int64_t lda,
+                                                       const float* __restrict__ x, float beta,
+                                                       float* __restrict__ y) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+    const int64_t r0 = wave * ROWS;
+    if (r0 >= M) return;
+    const int nr = (int)std::min<int64_t>(ROWS, M - r0);
+    float acc[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) acc[r] = 0.f;
+
+    int64_t done = 0;
+    if (VEC) {
+        const int64_t n4 = N >> 2;
+        const float4* x4 = reinterpret_cast<const float4*>(x);
+        const float4* a4[ROWS];
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r)
+            a4[r] = reinterpret_cast<const float4*>(A + (r0 + std::min(r, nr - 1)) * lda);
+        int64_t j = lane;
+        for (; j + 64 * (UNROLL - 1) < n4; j += 64 * UNROLL) {
+            float4 xv[UNROLL];
+            float4 av[ROWS][UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) xv[u] = x4[j + 64 * u];
+#pragma unroll
+            for (int r = 0; r < ROWS; ++r)
+#pragma unroll
+                for (int u = 0; u < UNROLL; ++u)
+                    av[r][u] = nt_load4(&a4[r][j + 64 * u]);
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u)
+#pragma unroll
+                for (int r = 0; r < ROWS; ++r) {
+                    acc[r] = fmaf(av[r][u].x, xv[u].x, acc[r]);
+                    acc[r] = fmaf(av[r][u].y, xv[u].y, acc[r]);
+                    acc[r] = fmaf(av[r][u].z, xv[u].z, acc[r]);
+                    acc[r] = fmaf(av[r][u].w, xv[u].w, acc[r]);
+                }
+        }
+        for (; j < n4; j += 64) {
+            float4 xv = x4[j];
+#pragma unroll
+            for (int r = 0; r < ROWS; ++r) {
+                float4 av = nt_load4(&a4[r][j]);
+                acc[r] = fmaf(av.x, xv.x, acc[r]);
+                acc[r] = fmaf(av.y, xv.y, acc[r]);
+                acc[r] = fmaf(av.z, xv.z, acc[r]);
+                acc[r] = fmaf(av.w, xv.w, acc[r]);
+            }
+        }
+        done = n4 << 2;
+    }
+    for (int64_t j = done + lane; j < N; j += 64) {
+        float xv = x[j];
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) {
+            if (r < nr) acc[r] = fmaf(A[(r0 + r) * lda + j], xv, acc[r]);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) acc[r] = wave_sum(acc[r]);
+    if (lane < nr) {
+        float s = acc[0];
+#pragma unroll
+        for (int r = 1; r < ROWS; ++r)
+            if (lane == r) s = acc[r];
+        float out = alpha * s;
+        if (beta != 0.f) out += beta * y[r0 + lane];
+        y[r0 + lane] = out;
+    }
+}
+
+constexpr int COL_ROWS = NT * 4;  // rows per workgroup in the column-major kernel
+constexpr int COL_UNROLL = 8;
+
+template <bool VEC>
+__global__ __launch_bounds__(NT) void sgemv_col_kernel(int64_t M, int64_t N, float alpha,
+                                                       const float* __restrict__ A, int64_t lda,
+                                                       const float* __restrict__ x, float beta,
+                                                       float* __restrict__ y, float* partial,
+                                                       int64_t mpad, unsigned* tickets,
+                                                       int64_t cols_per_chunk, int nchunks) {
+    const int64_t rb = blockIdx.x, ch = blockIdx.y;
+    const int64_t r = rb * COL_ROWS + threadIdx.x * 4;
+    const int64_t c0 = ch * cols_per_chunk;
+    const int64_t c1 = std::min<int64_t>(N, c0 + cols_per_chunk);
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    if (r < M) {
+        if (VEC) {  // M % 4 == 0, lda % 4 == 0, A 16-byte aligned: r..r+3 all valid
+            const float* ap = A + r;
+            int64_t j = c0;
+            for (; j + COL_UNROLL <= c1; j += COL_UNROLL) {
+                float4 av[COL_UNROLL];
+                float xv[COL_UNROLL];
+#pragma unroll
+                for (int u = 0; u < COL_UNROLL; ++u) {
+                    av[u] = nt_load4(reinterpret_cast<const float4*>(ap + (j + u) * lda));
+                    xv[u] = x[j + u];
+                }
+#pragma unroll
+                for (int u = 0; u < COL_UNROLL; ++u) {
+                    acc[0] = fmaf(av[u].x, xv[u], acc[0]);
+                    acc[1] = fmaf(av[u].y, xv[u], acc[1]);
+                    acc[2] = fmaf(av[u].z, xv[u], acc[2]);
+                    acc[3] = fmaf(av[u].w, xv[u], acc[3]);
+                }
+            }
+            for (; j < c1; ++j) {
+                float4 av = *reinterpret_cast<const float4*>(ap + j * lda);
+                float xv = x[j];
+                acc[0] = fmaf(av.x, xv, acc[0]);
+                acc[1] = fmaf(av.y, xv, acc[1]);
+                acc[2] = fmaf(av.z, xv, acc[2]);
+                acc[3] = fmaf(av.w, xv, acc[3]);
+            }
+        } else {
+            const int nv = (int)std::min<int64_t>(4, M - r);
+            for (int64_t j = c0; j < c1; ++j) {
+                const float xv = x[j];
+                const float* ap = A + j * lda + r;
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (i < nv) acc[i] = fmaf(ap[i], xv, acc[i]);
+            }
+        }
+    }
+    const int nv = r < M ? (int)std::min<int64_t>(4, M - r) : 0;
+    if (nchunks == 1) {
+        for (int i = 0; i < nv; ++i) {
+            float out = alpha * acc[i];
+            if (beta != 0.f) out += beta * y[r + i];
+            y[r + i] = out;
+        }
+        return;
+    }
+    // Partial sums of this column chunk, then the last workgroup of the row
+    // block adds the chunks in order 0..nchunks-1.
+    if (nv > 0) {
+        float* pp = partial + ch * mpad + r;
+        for (int i = 0; i < 4; ++i) pp[i] = acc[i];
+    }
+    __threadfence();
+    __syncthreads();
+    __shared__ unsigned s_last;
+    if (threadIdx.x == 0) s_last = atomicAdd(&tickets[rb], 1u) == (unsigned)(nchunks - 1);
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    if (nv > 0) {
+        float s[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int c = 0; c < nchunks; ++c) {
+            const float* pp = partial + (int64_t)c * mpad + r;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) s[i] += __builtin_nontemporal_load(pp + i);
+        }
+        for (int i = 0; i < nv; ++i) {
+            float out = alpha * s[i];
+            if (beta != 0.f) out += beta * y[r + i];
+            y[r + i] = out;
+        }
+    }
+    if (threadIdx.x == 0) tickets[rb] = 0u;
+}
+
+}  // namespace
+
+omb_status launch_sgemv_row(int64_t M, int64_t N, float alpha, const float* A, int64_t lda,
+                            const float* x, float beta, float* y, hipStream_t s) {
+    if (M <= 0) return OMB_OK;
+    const int64_t waves = ceil_div(M, ROWS);
+    const int64_t blocks = ceil_div(waves, NT / 64);
+    if (blocks > 0x7fffffffLL) {
+        set_error("sgemv: too many rows");
+        return OMB_EUNSUPPORTED;
+    }
+    const bool vec = ((reinterpret_cast<uintptr_t>(A) & 15) == 0) &&
+                     ((reinterpret_cast<uintptr_t>(x) & 15) == 0) && (lda % 4 == 0);
+    if (vec)
+        hipLaunchKernelGGL(sgemv_row_kernel<true>, dim3((unsigned)blocks), dim3(NT), 0, s, M, N,
+                           alpha, A, lda, x, beta, y);
+    else
+        hipLaunchKernelGGL(sgemv_row_kernel<false>, dim3((unsigned)blocks), dim3(NT), 0, s, M, N,
+                           alpha, A, lda, x, beta, y);
+    OMB_HIP_TRY(hipGetLastError());
+    return OMB_OK;
+}
+
+omb_status launch_sgemv_col(int64_t M, int64_t N, float alpha, const float* A, int64_t lda,
+                            const float* x, float beta, float* y, hipStream_t s) {
+    if (M <= 0) return OMB_OK;
+    const int64_t nrb = ceil_div(M, COL_ROWS);
+    // Enough workgroups to cover the chip (~1024), at least 64 columns each.
+    int64_t nchunks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(1024, nrb), ceil_div(N, 64)));
+    if (N == 0) nchunks = 1;
+    const int64_t cpc = N > 0 ? ceil_div(N, nchunks) : 0;
+    nchunks = N > 0 ? ceil_div(N, cpc) : 1;
+    if (nrb > Workspace::kTickets || nrb > 0x7fffffffLL) {
+        set_error("sgemv: too many row blocks");
+        return OMB_EUNSUPPORTED;
+    }
+    const int64_t mpad = round_up(M, 4);
+    Workspace* ws = nullptr;
+    OMB_TRY(get_workspace(s, nchunks > 1 ? (size_t)(nchunks * mpad) * sizeof(float) : 0, &ws));
+    const bool vec = ((reinterpret_cast<uintptr_t>(A) & 15) == 0) && (lda % 4 == 0) && (M % 4 == 0);
+    dim3 grid((unsigned)nrb, (unsigned)nchunks);
+    float* part = reinterpret_cast<float*>(ws->buf);
+    if (vec)
+        hipLaunchKernelGGL(sgemv_col_kernel<true>, grid, dim3(NT), 0, s, M, N, alpha, A, lda, x,
+                           beta, y, part, mpad, ws->tickets, cpc, (int)nchunks);
+    else
+        hipLaunchKernelGGL(sgemv_col_kernel<false>, grid, dim3(NT), 0, s, M, N, alpha, A, lda, x,
+                           beta, y, part, mpad, ws->tickets, cpc, (int)nchunks);
+    OMB_HIP_TRY(hipGetLastError());
+    return OMB_OK;
+}
+
+}  // namespace omb

@@ -1,0 +1,99 @@
+"""The C-ABI library loads, exports every declared symbol, and rejects bad
+arguments before touching the device.  CPU only: no call here reaches a
+kernel launch."""
+import ctypes
+
+import pytest
+
+import helpers
+
+
+def test_library_exports_every_header_symbol(ompblas):
+    lib = ompblas.lib()
+    names = ompblas.header_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert missing == []
+
+
+def test_every_header_symbol_has_a_ctypes_signature(ompblas):
+    assert sorted(ompblas.header_functions()) == sorted(ompblas._SIGS)
+
+
+def test_version(ompblas):
+    assert ompblas.lib().omb_version().decode().startswith("ompblas")
+
+
+def _sgemm(ompblas, **kw):
+    args = dict(M=4, N=4, K=4, alpha=1.0, A=16, rsA=4, csA=1, B=16, rsB=4, csB=1, beta=1.0,
+                C=16, rsC=4, csC=1, mode=0)
+    args.update(kw)
+    return ompblas.sgemm(**args)
+
+
+@pytest.mark.parametrize("kw,status", [
+    (dict(M=-1), -1), (dict(N=-2), -1), (dict(K=-3), -1),
+    (dict(C=None), -1), (dict(A=None), -1), (dict(B=None), -1),
+    (dict(rsA=3), -1),                   # ld < K for a row-major A
+    (dict(rsC=2, csC=2), -2),            # neither stride is 1
+    (dict(rsB=0), -1),                   # stride 0
+    (dict(mode=7), -1),
+])
+def test_sgemm_rejects_bad_arguments(ompblas, kw, status):
+    assert _sgemm(ompblas, **kw) == status
+    assert ompblas.last_error() != ""
+
+
+def test_sgemm_empty_is_ok_without_device(ompblas):
+    assert _sgemm(ompblas, M=0) == 0
+    assert _sgemm(ompblas, N=0, C=None) == 0
+    assert ompblas.last_error() == ""
+
+
+def test_sgemv_and_dot_reject_bad_arguments(ompblas):
+    assert ompblas.sgemv(-1, 4, 1.0, 16, 4, 1, 16, 0.0, 16) == -1
+    assert ompblas.sgemv(4, 4, 1.0, None, 4, 1, 16, 0.0, 16) == -1
+    assert ompblas.sgemv(4, 4, 1.0, 16, 4, 1, 16, 0.0, None) == -1
+    assert ompblas.sgemv(4, 4, 1.0, 16, 2, 1, 16, 0.0, 16) == -1     # ld < N
+    assert ompblas.sgemv(0, 4, 1.0, None, 4, 1, None, 0.0, None) == 0
+    assert ompblas.sdot(-1, 16, 16, 16) == -1
+    assert ompblas.sdot(4, 16, 16, None) == -1
+    assert ompblas.sdot(4, None, 16, 16) == -1
+    assert "omb_sdot" in ompblas.last_error()
+
+
+def test_check_and_fill_reject_bad_arguments(ompblas):
+    lib = ompblas.lib()
+    assert lib.omb_fill_uniform(None, 10, 0, 0, 0, None) == -1
+    assert lib.omb_fill_uniform(None, -1, 0, 0, 0, None) == -1
+    assert lib.omb_check_dgemm(-1, 1, 1, 16, 1, 1, 16, 1, 1, None, 1, 1, 16, None) == -1
+    assert lib.omb_check_errnorm(2, 2, 16, 2, 1, 16, None, None) == -1
+
+
+def test_torch_mirror_shape_errors(ompblas):
+    """amt::mtm/mtv/dot_prod throw std::runtime_error at closure creation on
+    non-matrix input or mismatched extents (include/mtm.hpp:234-250,
+    include/mtv.hpp:126-144, include/dot.hpp:196-214)."""
+    import torch
+    a = torch.zeros(3, 4)
+    b = torch.zeros(5, 2)
+    c = torch.zeros(3, 2)
+    with pytest.raises(RuntimeError, match="dimension mismatch"):
+        ompblas.mtm(c, a, b)
+    with pytest.raises(RuntimeError, match="must be the matrices"):
+        ompblas.mtm(c, torch.zeros(3), b)
+    with pytest.raises(RuntimeError, match="dimension mismatch"):
+        ompblas.mtv(torch.zeros(3), a, torch.zeros(5))
+    with pytest.raises(RuntimeError, match="same size"):
+        ompblas.dot_prod(torch.zeros(1), torch.zeros(4), torch.zeros(5))
+
+
+def test_oracle_is_not_linked_into_the_product():
+    """The product library must not depend on the oracle."""
+    import subprocess
+    out = subprocess.run(["ldd", helpers.graft.load_package().LIB_PATH], capture_output=True,
+                         text=True).stdout
+    assert "oracle" not in out
+    syms = subprocess.run(["nm", "-D", helpers.graft.load_package().LIB_PATH],
+                          capture_output=True, text=True).stdout
+    assert "oracle_" not in syms

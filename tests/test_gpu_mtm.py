@@ -1,0 +1,175 @@
+"""mtm (SGEMM) on the GPU through the C-ABI against the oracle.
+
+Integer-exact cases mirror test/test.mtm.cpp:30-460 (8 (C,A,B) layout
+combos, sizes 2..31, rand()%100, C = 0); floating-point cases compare the
+max- and Frobenius-relative error against an FP64 product with the oracle's
+(the reference algorithm's FP32 error on the same inputs): GPU <= 2x oracle.
+"""
+import numpy as np
+import pytest
+import torch
+
+import helpers
+from helpers import oracle_mtm, rand100, rel_errors
+
+pytestmark = pytest.mark.gpu
+
+LAYOUTS = ["FFF", "FFL", "FLF", "LFF", "FLL", "LFL", "LLF", "LLL"]
+MODES = [0, 1, 2]  # plain FP32, split-BF16 (6 products), split-BF16 (3 products)
+
+
+def to_dev(a: np.ndarray, dev) -> torch.Tensor:
+    """Device copy keeping the numpy array's storage order (strides)."""
+    if a.flags.f_contiguous and not a.flags.c_contiguous:
+        return torch.from_numpy(np.ascontiguousarray(a.T)).to(dev).t()
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def run_mtm(ompblas, c, a, b, mode, alpha=1.0, beta=1.0):
+    M, K = a.shape
+    N = b.shape[1]
+    st = ompblas.sgemm(M, N, K, alpha, a.data_ptr(), a.stride(0), a.stride(1), b.data_ptr(),
+                       b.stride(0), b.stride(1), beta, c.data_ptr(), c.stride(0), c.stride(1),
+                       mode)
+    ompblas.check(st)
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("combo", LAYOUTS)
+def test_mtm_integer_exact_reference_cases(dev, ompblas, combo, mode):
+    """Values 0..99 are exact in one BF16 term, so every mode is exact here."""
+    order = {"F": "F", "L": "C"}
+    oc, oa, ob = (order[ch] for ch in combo)
+    for sz in range(2, 32):
+        a = rand100((sz, sz), oa)
+        b = rand100((sz, sz), ob)
+        c = np.zeros((sz, sz), np.float32, order=oc)
+        oracle_mtm(c, a, b)
+        tc = to_dev(np.zeros((sz, sz), np.float32, order=oc), dev)
+        run_mtm(ompblas, tc, to_dev(a, dev), to_dev(b, dev), mode)
+        got = tc.cpu().numpy()
+        assert np.array_equal(got, c), (combo, sz, mode)
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("shape", [(1, 1, 1), (1, 300, 7), (300, 1, 7), (7, 5, 1),
+                                   (129, 131, 17), (257, 100, 333), (64, 64, 2048),
+                                   (1000, 37, 129)])
+def test_mtm_ragged_integer_exact_accumulate(dev, ompblas, shape, mode):
+    """C += A*B with C != 0 (the reference accumulates, simd_loop.hpp:169,187)."""
+    M, N, K = shape
+    a = rand100((M, K), "C") % 7
+    b = rand100((K, N), "F") % 13
+    c0 = rand100((M, N), "C")
+    want = c0.astype(np.int64) + a.astype(np.int64) @ b.astype(np.int64)
+    assert np.abs(want).max() < 2 ** 24
+    tc = to_dev(c0, dev)
+    run_mtm(ompblas, tc, to_dev(a, dev), to_dev(b, dev), mode)
+    assert np.array_equal(tc.cpu().numpy().astype(np.int64), want)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_mtm_leading_dimension_and_unaligned_views(dev, ompblas, mode):
+    """Views with ld > extent, odd offsets (no 16-byte alignment) and odd ld."""
+    big_a = torch.from_numpy(rand100((301, 203), "C")).to(dev)
+    big_b = torch.from_numpy(rand100((157, 181), "C") % 9).to(dev).t()   # col-major 181x157
+    big_c = torch.zeros((97, 389), device=dev)
+    a = big_a[1:90, 3:120]           # 89 x 117, row stride 203, offset 206 floats
+    b = big_b[3:120, 5:150]          # 117 x 145, col stride 181
+    c = big_c[2:91, 7:152]           # 89 x 145, row stride 389
+    run_mtm(ompblas, c, a, b, mode)
+    want = a.cpu().numpy().astype(np.int64) @ b.cpu().numpy().astype(np.int64)
+    assert np.array_equal(c.cpu().numpy().astype(np.int64), want)
+    assert float(big_c[0].abs().sum()) == 0.0 and float(big_c[:, :7].abs().sum()) == 0.0
+
+
+def test_mtm_alpha_beta_and_k_zero(dev, ompblas):
+    a = torch.from_numpy(rand100((33, 20), "C")).to(dev)
+    b = torch.from_numpy(rand100((20, 40), "C")).to(dev)
+    c0 = torch.from_numpy(rand100((33, 40), "C")).to(dev)
+    c = c0.clone()
+    run_mtm(ompblas, c, a, b, 0, alpha=2.0, beta=0.5)
+    want = 2.0 * (a.double() @ b.double()) + 0.5 * c0.double()
+    assert torch.equal(c.double(), want)
+    c = c0.clone()
+    c[0, 0] = float("nan")
+    run_mtm(ompblas, c, a, b, 0, alpha=1.0, beta=0.0)    # beta = 0 never reads C
+    assert torch.equal(c.double(), a.double() @ b.double())
+    c = c0.clone()
+    ompblas.check(ompblas.sgemm(33, 40, 0, 1.0, a.data_ptr(), 20, 1, b.data_ptr(), 40, 1, 3.0,
+                                c.data_ptr(), 40, 1, 0))
+    assert torch.equal(c, 3.0 * c0)
+
+
+def _fp_case(dev, ompblas, M, N, K, seed, c_order="C", a_order="C", b_order="C"):
+    a = helpers.rng_uniform(M * K, seed, 0).reshape((M, K), order=a_order)
+    b = helpers.rng_uniform(K * N, seed, 1).reshape((K, N), order=b_order)
+    truth = a.astype(np.float64) @ b.astype(np.float64)
+    c_ref = np.zeros((M, N), np.float32, order=c_order)
+    oracle_mtm(c_ref, a, b)
+    ref = rel_errors(c_ref, truth)
+    ta, tb = to_dev(a, dev), to_dev(b, dev)
+    out = {}
+    for mode in MODES:
+        tc = to_dev(np.zeros((M, N), np.float32, order=c_order), dev)
+        run_mtm(ompblas, tc, ta, tb, mode)
+        out[mode] = rel_errors(tc.cpu().numpy(), truth)
+    return ref, out
+
+
+def test_mtm_config1_error_vs_reference(dev, ompblas):
+    """Config 1: row-major 1024^3, U[-1,1) from seed 0, C = 0.  Plain FP32 and
+    split-BF16 (6 products): max- and Frobenius-relative error <= 2x the
+    reference algorithm's (oracle, this host's KC)."""
+    ref, out = _fp_case(dev, ompblas, 1024, 1024, 1024, 0)
+    print("oracle", ref, "gpu", out)
+    for mode in (0, 1):
+        assert out[mode][0] <= 2 * ref[0], (mode, out[mode], ref)
+        assert out[mode][1] <= 2 * ref[1], (mode, out[mode], ref)
+    assert out[2][1] < 1e-4   # 3-product variant: only a loose sanity bound
+
+
+@pytest.mark.parametrize("orders", [("F", "F", "F"), ("C", "F", "C"), ("F", "C", "F")])
+def test_mtm_mixed_layout_error(dev, ompblas, orders):
+    ref, out = _fp_case(dev, ompblas, 777, 555, 1999, 13, *orders)
+    for mode in (0, 1):
+        assert out[mode][1] <= 2 * ref[1], (mode, out[mode], ref)
+
+
+def test_mtm_deterministic(dev, ompblas):
+    M = N = K = 700
+    a = torch.empty((M, K), device=dev)
+    b = torch.empty((K, N), device=dev)
+    ompblas.fill_uniform(a, 3, 0)
+    ompblas.fill_uniform(b, 3, 1)
+    for mode in MODES:
+        outs = []
+        for _ in range(2):
+            c = torch.zeros((M, N), device=dev)
+            run_mtm(ompblas, c, a, b, mode)
+            outs.append(c)
+        assert torch.equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("layout", ["row", "col"])
+def test_mtm_8192_slab_error_vs_reference(dev, ompblas, layout):
+    """Config 2 (8192^3, seed 1) in both layouts: error on a slab of 64 rows,
+    GPU vs the oracle's on the same rows (the oracle's per-element result does
+    not depend on which other rows are computed)."""
+    n = 8192
+    order = "C" if layout == "row" else "F"
+    a = helpers.rng_uniform(n * n, 1, 0).reshape((n, n), order=order)
+    b = helpers.rng_uniform(n * n, 1, 1).reshape((n, n), order=order)
+    rows = slice(4000, 4064)
+    truth = a[rows].astype(np.float64) @ b.astype(np.float64)
+    c_ref = np.zeros((64, n), np.float32, order=order)
+    oracle_mtm(c_ref, np.asarray(a[rows], order=order), b)
+    ref = rel_errors(c_ref, truth)
+    ta, tb = to_dev(a, dev), to_dev(b, dev)
+    del a, b
+    for mode in (0, 1):
+        tc = to_dev(np.zeros((n, n), np.float32, order=order), dev)
+        run_mtm(ompblas, tc, ta, tb, mode)
+        got = rel_errors(tc[rows].cpu().numpy(), truth)
+        print(layout, mode, "gpu", got, "oracle", ref)
+        assert got[0] <= 2 * ref[0] and got[1] <= 2 * ref[1], (mode, got, ref)

@@ -1,0 +1,166 @@
+"""mtv / vtm (SGEMV) and dot (SDOT) on the GPU through the C-ABI against
+the oracle: integer-exact reference cases (test/test.mtv.cpp, test.vtm.cpp,
+test.dot_prod.cpp) and FP64-error bounds at config-2 sizes."""
+import numpy as np
+import pytest
+import torch
+
+import helpers
+from helpers import oracle_dot, oracle_mtv, rand100
+
+pytestmark = pytest.mark.gpu
+
+MTV_SIZES = list(range(2, 64)) + list(range(64, 512, 11)) + [511]
+
+
+def to_dev(a, dev):
+    if a.ndim == 2 and a.flags.f_contiguous and not a.flags.c_contiguous:
+        return torch.from_numpy(np.ascontiguousarray(a.T)).to(dev).t()
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+@pytest.mark.parametrize("order", ["F", "C"])
+def test_mtv_integer_exact_reference_cases(dev, ompblas, order):
+    """test/test.mtv.cpp:29-126 via the torch mirror amt.mtv(c, A, v)():
+    first order accumulates into c, last order overwrites it."""
+    for sz in MTV_SIZES:
+        a = rand100((sz, sz), order)
+        x = rand100((sz,))
+        y0 = rand100((sz,))
+        y = y0.copy()
+        oracle_mtv(y, a, x)
+        ty = torch.from_numpy(y0).to(dev)
+        ompblas.mtv(ty, to_dev(a, dev), torch.from_numpy(x).to(dev))()
+        assert np.array_equal(ty.cpu().numpy(), y), (order, sz)
+
+
+@pytest.mark.parametrize("order", ["F", "C"])
+def test_vtm_integer_exact_true_transpose(dev, ompblas, order):
+    """test/test.vtm.cpp:29-125: c = A^T v; rectangular shapes as well."""
+    shapes = [(s, s) for s in MTV_SIZES[::2]] + [(5, 9), (40, 3), (1, 17), (17, 1), (300, 1000)]
+    for m, n in shapes:
+        a = rand100((m, n), order)
+        x = rand100((m,))
+        ty = torch.zeros(n, device=dev)
+        ompblas.vtm(ty, to_dev(a, dev), torch.from_numpy(x).to(dev))()
+        want = a.astype(np.int64).T @ x.astype(np.int64)
+        assert np.array_equal(ty.cpu().numpy().astype(np.int64), want), (order, m, n)
+
+
+@pytest.mark.parametrize("order", ["F", "C"])
+def test_gemv_unaligned_and_ld(dev, ompblas, order):
+    big = torch.from_numpy(rand100((403, 517), order)).to(dev) if order == "C" else \
+        to_dev(rand100((403, 517), "F"), dev)
+    a = big[3:301, 1:389]                  # offsets break 16-byte alignment
+    x = torch.from_numpy(rand100((389,))).to(dev)
+    y = torch.zeros(298, device=dev)
+    ompblas.check(ompblas.sgemv(298, 389, 1.0, a.data_ptr(), a.stride(0), a.stride(1),
+                                x.data_ptr(), 0.0, y.data_ptr()))
+    want = a.cpu().numpy().astype(np.int64) @ x.cpu().numpy().astype(np.int64)
+    assert np.array_equal(y.cpu().numpy().astype(np.int64), want)
+
+
+@pytest.mark.parametrize("order", ["F", "C"])
+def test_gemv_config2_error_vs_reference(dev, ompblas, order):
+    """Config 2: A 16384 x 16384 and x from seed 1, both layouts.  GPU error
+    vs FP64 <= 2x the oracle's (reference algorithm, this host's blocking)."""
+    n = 16384
+    a = helpers.rng_uniform(n * n, 1, 0).reshape((n, n), order=order)
+    x = helpers.rng_uniform(n, 1, 1)
+    ta, tx = to_dev(a, dev), torch.from_numpy(x).to(dev)
+    y64 = torch.empty(n, dtype=torch.float64, device=dev)
+    ompblas.check(ompblas.lib().omb_check_dgemv(n, n, ta.data_ptr(), ta.stride(0), ta.stride(1),
+                                                tx.data_ptr(), None, y64.data_ptr(), None))
+    truth = y64.cpu().numpy()
+    # validate the FP64 check kernel on 64 rows against numpy FP64
+    rows = np.arange(0, n, 257)
+    np.testing.assert_allclose(truth[rows], a[rows].astype(np.float64) @ x.astype(np.float64),
+                               rtol=1e-12, atol=1e-12)
+    y_ref = np.zeros(n, np.float32)
+    oracle_mtv(y_ref, a, x)
+    ref = helpers.rel_errors(y_ref, truth)
+    ty = torch.zeros(n, device=dev)
+    ompblas.check(ompblas.sgemv(n, n, 1.0, ta.data_ptr(), ta.stride(0), ta.stride(1),
+                                tx.data_ptr(), 0.0, ty.data_ptr()))
+    got = helpers.rel_errors(ty.cpu().numpy(), truth)
+    print(order, "gpu", got, "oracle", ref)
+    assert got[0] <= 2 * ref[0] and got[1] <= 2 * ref[1]
+
+
+def test_gemv_deterministic_and_beta(dev, ompblas):
+    M, N = 5000, 70000
+    a = torch.empty((N, M), device=dev).t()     # column-major, split over column chunks
+    ompblas.fill_uniform(a.t(), 8, 0)
+    x = torch.empty(N, device=dev)
+    ompblas.fill_uniform(x, 8, 1)
+    y0 = torch.empty(M, device=dev)
+    ompblas.fill_uniform(y0, 8, 2)
+    outs = []
+    for _ in range(3):
+        y = y0.clone()
+        ompblas.check(ompblas.sgemv(M, N, 0.5, a.data_ptr(), a.stride(0), a.stride(1),
+                                    x.data_ptr(), 2.0, y.data_ptr()))
+        outs.append(y)
+    assert torch.equal(outs[0], outs[1]) and torch.equal(outs[0], outs[2])
+    want = 0.5 * (a.double() @ x.double()) + 2.0 * y0.double()
+    assert torch.allclose(outs[0].double(), want, rtol=0, atol=1e-3)
+
+
+# ----------------------------------------------------------------- dot ----
+DOT_SIZES = list(range(2, 200)) + list(range(200, (1 << 17) + 2, 1009)) + [(1 << 17) + 1]
+
+
+def test_dot_closed_form(dev, ompblas):
+    """test/test.dot_prod.cpp:10-28: a = 1..s, a.a = s(s+1)(2s+1)/6, within the
+    summation bound (exact while the sum stays below 2^24)."""
+    full = torch.arange(1, (1 << 17) + 2, dtype=torch.float32, device=dev)
+    r = torch.zeros(1, device=dev)
+    u = 2.0 ** -24
+    for s in DOT_SIZES:
+        a = full[:s]
+        ompblas.dot_prod(r, a, a)()
+        got = float(r.item())
+        exact = s * (s + 1) * (2 * s + 1) // 6
+        if exact < (1 << 24):
+            assert got == exact, s
+        else:
+            assert abs(got - exact) <= (s * u / (1 - s * u)) * exact, s
+
+
+def test_dot_config2_error_vs_reference(dev, ompblas):
+    """Config 2: two 2^28 vectors from seed 1 (FP64 cross-workgroup combine)."""
+    n = 1 << 28
+    x = torch.empty(n, device=dev)
+    y = torch.empty(n, device=dev)
+    ompblas.fill_uniform(x, 1, 0)
+    ompblas.fill_uniform(y, 1, 1)
+    r64 = torch.empty(1, dtype=torch.float64, device=dev)
+    ompblas.check(ompblas.lib().omb_check_ddot(n, x.data_ptr(), y.data_ptr(), r64.data_ptr(), None))
+    truth = float(r64.item())
+    r = torch.zeros(1, device=dev)
+    ompblas.dot_prod(r, x, y)()
+    got = float(r.item())
+    xs, ys = x.cpu().numpy(), y.cpu().numpy()
+    ref = oracle_dot(xs, ys)
+    t_np = float(np.dot(xs.astype(np.float64), ys.astype(np.float64)))
+    assert abs(truth - t_np) <= 1e-9 * abs(t_np) + 1e-6
+    absdot = float(np.dot(np.abs(xs).astype(np.float64), np.abs(ys).astype(np.float64)))
+    print("dot gpu", got, "oracle", ref, "fp64", truth)
+    # a single scalar's error can be small by chance: bound it by the larger
+    # of 2x the oracle's and a fixed fraction of the summation bound
+    bound = max(2 * abs(ref - truth), 1e-3 * (n * 2.0 ** -24) * absdot)
+    assert abs(got - truth) <= bound
+    ompblas.dot_prod(r, x, y)()
+    assert float(r.item()) == got     # bitwise repeatable
+
+
+def test_dot_edge_cases(dev, ompblas):
+    r = torch.full((1,), 5.0, device=dev)
+    ompblas.check(ompblas.sdot(0, 0, 0, r.data_ptr()))
+    assert float(r.item()) == 0.0
+    v = torch.arange(1, 40, dtype=torch.float32, device=dev)
+    for off in range(1, 4):                      # unaligned starts
+        a = v[off:off + 30]
+        ompblas.dot_prod(r, a, a)()
+        want = sum(float(t) ** 2 for t in range(off + 1, off + 31))
+        assert float(r.item()) == want
