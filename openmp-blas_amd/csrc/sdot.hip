@@ -4,11 +4,13 @@
 // regimes (serial simd, parallel-for reduction, blocked parallel reduction)
 // all reduce FP32 lane partials into per-thread and then per-team sums.
 // Here: a grid-stride loop of 128-bit loads (4 float4 of x and of y in flight
-// per lane) into FP32 lane sums, a fixed butterfly per wavefront, a fixed
-// LDS tree per workgroup, one partial per workgroup, and the last workgroup
-// (ticket counter) adds the partials in index order -- in FP64 when
-// n >= 2^26, in FP32 below.  The grid depends on n only, so the result is
-// bitwise reproducible.
+// per lane) into FP32 lane sums, a fixed butterfly per wavefront, the four
+// wavefront sums in order, one partial per workgroup; a second one-workgroup
+// kernel adds the partials in a fixed order -- in FP64 when n >= 2^26, in
+// FP32 below.  (A second launch instead of a last-workgroup ticket: on this
+// GPU an agent-scope fence writes back and invalidates the XCD's L2, which
+// costs more than the launch.)  The grid depends on n only, so the result
+// is bitwise reproducible.
 #include <algorithm>
 
 #include "common.hpp"
@@ -32,27 +34,35 @@ __device__ inline float dot4(float4 a, float4 b, float acc) {
 template <bool VEC>
 __global__ __launch_bounds__(NT) void sdot_kernel(int64_t n, const float* __restrict__ x,
                                                   const float* __restrict__ y, float* result,
-                                                  double* partial, unsigned* ticket, int f64) {
+                                                  double* partial) {
     const int64_t gtid = (int64_t)blockIdx.x * NT + threadIdx.x;
     const int64_t nthr = (int64_t)gridDim.x * NT;
+    const int lane = threadIdx.x & 63;
     float acc = 0.f;
     int64_t tail = 0;
     if (VEC) {
+        // Each wavefront streams one contiguous segment: per step 4 KB of x
+        // and 4 KB of y (4 x 64 lanes x 16 bytes).
         const int64_t n4 = n >> 2;
         const float4* x4 = reinterpret_cast<const float4*>(x);
         const float4* y4 = reinterpret_cast<const float4*>(y);
-        int64_t i = gtid;
-        for (; i + nthr * (UNROLL - 1) < n4; i += nthr * UNROLL) {
+        const int64_t waves = (int64_t)gridDim.x * (NT / 64);
+        const int64_t wave = (int64_t)blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+        const int64_t per = round_up(ceil_div(n4, waves), 64 * UNROLL);
+        const int64_t beg = wave * per;
+        const int64_t end = beg + per < n4 ? beg + per : n4;
+        int64_t i = beg + lane;
+        for (; i + 64 * (UNROLL - 1) < end; i += 64 * UNROLL) {
             float4 a[UNROLL], b[UNROLL];
 #pragma unroll
             for (int u = 0; u < UNROLL; ++u) {
-                a[u] = nt_load4(&x4[i + nthr * u]);
-                b[u] = nt_load4(&y4[i + nthr * u]);
+                a[u] = nt_load4(&x4[i + 64 * u]);
+                b[u] = nt_load4(&y4[i + 64 * u]);
             }
 #pragma unroll
             for (int u = 0; u < UNROLL; ++u) acc = dot4(a[u], b[u], acc);
         }
-        for (; i < n4; i += nthr) acc = dot4(x4[i], y4[i], acc);
+        for (; i < end; i += 64) acc = dot4(x4[i], y4[i], acc);
         tail = n4 << 2;
     }
     for (int64_t i = tail + gtid; i < n; i += nthr) acc = fmaf(x[i], y[i], acc);
@@ -62,36 +72,26 @@ __global__ __launch_bounds__(NT) void sdot_kernel(int64_t n, const float* __rest
     acc = wave_sum(acc);
     if ((threadIdx.x & 63) == 0) s_wave[threadIdx.x >> 6] = acc;
     __syncthreads();
-    if (gridDim.x == 1) {
-        if (threadIdx.x == 0) {
-            float s = s_wave[0];
-            for (int w = 1; w < NT / 64; ++w) s += s_wave[w];
-            *result = s;
-        }
-        return;
-    }
     if (threadIdx.x == 0) {
         float s = s_wave[0];
         for (int w = 1; w < NT / 64; ++w) s += s_wave[w];
-        partial[blockIdx.x] = (double)s;
+        if (gridDim.x == 1) *result = s;
+        else partial[blockIdx.x] = (double)s;
     }
-    __threadfence();
-    __syncthreads();
-    __shared__ unsigned s_last;
-    if (threadIdx.x == 0) s_last = atomicAdd(ticket, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    // Last workgroup: thread t adds partials t, t+NT, ... in order, then a
-    // fixed tree over the threads.
+}
+
+// One workgroup adds the per-workgroup partials: thread t takes partials
+// t, t+NT, ... in order, then a fixed tree -- in FP64 or in FP32.
+__global__ __launch_bounds__(NT) void sdot_combine_kernel(const double* __restrict__ partial,
+                                                          int count, int f64, float* result) {
     __shared__ double s_red[NT];
     if (f64) {
         double s = 0.0;
-        for (int64_t b = threadIdx.x; b < gridDim.x; b += NT) s += __builtin_nontemporal_load(&partial[b]);
+        for (int b = threadIdx.x; b < count; b += NT) s += partial[b];
         s_red[threadIdx.x] = s;
     } else {
         float s = 0.f;
-        for (int64_t b = threadIdx.x; b < gridDim.x; b += NT) s += (float)__builtin_nontemporal_load(&partial[b]);
+        for (int b = threadIdx.x; b < count; b += NT) s += (float)partial[b];
         s_red[threadIdx.x] = (double)s;
     }
     __syncthreads();
@@ -102,10 +102,7 @@ __global__ __launch_bounds__(NT) void sdot_kernel(int64_t n, const float* __rest
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        *result = (float)s_red[0];
-        *ticket = 0u;
-    }
+    if (threadIdx.x == 0) *result = (float)s_red[0];
 }
 
 __global__ void zero_scalar_kernel(float* r) { *r = 0.f; }
@@ -128,10 +125,13 @@ omb_status launch_sdot(int64_t n, const float* x, const float* y, float* result,
     double* part = reinterpret_cast<double*>(ws->buf);
     if (vec)
         hipLaunchKernelGGL(sdot_kernel<true>, dim3((unsigned)blocks), dim3(NT), 0, s, n, x, y,
-                           result, part, ws->tickets, f64);
+                           result, part);
     else
         hipLaunchKernelGGL(sdot_kernel<false>, dim3((unsigned)blocks), dim3(NT), 0, s, n, x, y,
-                           result, part, ws->tickets, f64);
+                           result, part);
+    if (blocks > 1)
+        hipLaunchKernelGGL(sdot_combine_kernel, dim3(1), dim3(NT), 0, s, part, (int)blocks, f64,
+                           result);
     OMB_HIP_TRY(hipGetLastError());
     return OMB_OK;
 }

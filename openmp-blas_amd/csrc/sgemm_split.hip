@@ -16,9 +16,11 @@
 //
 // Pass 2 (sgemm_bf16x3_kernel): a 256-thread workgroup owns a 128x128 tile of
 // C; four wavefronts each own 64x64 (2x2 blocks of v_mfma_f32_32x32x16_bf16).
-// Slices are staged through LDS (double buffered).  a0b0 accumulates into
-// one register set and the smaller products into a second one, so adding
-// the small terms never rounds at the magnitude of the running sum.
+// Slices are prefetched two ahead into registers and staged through LDS
+// (double buffered, bank-swizzled).  All products of 8 slices (128 k) are
+// accumulated by the MFMAs into an inner register tile, smallest first, which
+// is then added into an outer tile with FP32 adds: the running sum over K
+// never absorbs MFMA rounding at its own magnitude.
 #include <algorithm>
 
 #include "common.hpp"
@@ -103,9 +105,49 @@ __device__ inline f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
 
+// LDS position of 16-byte chunk h (0/1) of slice row r: the chunk index is
+// flipped by bits 2 and 3 of the row so that the 8 or 16 lanes a ds_read_b128
+// serves per cycle (rows r..r+15, same h) hit distinct banks.
+__device__ inline int lds_chunk(int r, int h) { return (r << 1) | (h ^ ((r >> 2) & 1) ^ ((r >> 3) & 1)); }
+
+constexpr int kFoldSlices = 8;  // inner accumulation spans 8 slices = 128 k
+constexpr int VA = SLICE_A / 8 / NT;  // 16-byte loads per thread per slice (3)
+constexpr int VB = SLICE_B / 8 / NT;
+constexpr int V = VA + VB;
+
+// One slice in flight is 6 uint4 per thread (3 of A, 3 of B), kept in named
+// registers (arrays of them were demoted to scratch by the compiler).
+#define OMB_GLOAD(kt, r0, r1, r2, r3, r4, r5)                            \
+    do {                                                                  \
+        const uint4* a_ = ga + (kt) * (SLICE_A / 8) + tid;                \
+        const uint4* b_ = gb + (kt) * (SLICE_B / 8) + tid;                \
+        r0 = a_[0];                                                       \
+        r1 = a_[NT];                                                      \
+        r2 = a_[2 * NT];                                                  \
+        r3 = b_[0];                                                       \
+        r4 = b_[NT];                                                      \
+        r5 = b_[2 * NT];                                                  \
+    } while (0)
+#define OMB_LSTORE(l, r0, r1, r2, r3, r4, r5)                             \
+    do {                                                                  \
+        (l)[q0] = r0;                                                     \
+        (l)[q1] = r1;                                                     \
+        (l)[q2] = r2;                                                     \
+        (l)[q3] = r3;                                                     \
+        (l)[q4] = r4;                                                     \
+        (l)[q5] = r5;                                                     \
+    } while (0)
+
+// Slice chunk q of A (0 .. 3*TM*2-1): plane q / (2*TM), row (q / 2) % TM,
+// half q % 2 (same for B after the A part) -> its swizzled LDS position.
+__device__ __forceinline__ int lds_index(int q) {
+    const int pl = q / (2 * TM), rem = q - pl * 2 * TM;
+    return pl * 2 * TM + lds_chunk(rem >> 1, rem & 1);
+}
+
 template <int NPROD>
 __global__ __launch_bounds__(NT, 2) void sgemm_bf16x3_kernel(MArgs p) {
-    __shared__ __attribute__((aligned(16))) __bf16 lds[2][SLICE_A + SLICE_B];
+    __shared__ __attribute__((aligned(16))) uint4 lds[2][(SLICE_A + SLICE_B) / 8];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -117,71 +159,77 @@ __global__ __launch_bounds__(NT, 2) void sgemm_bf16x3_kernel(MArgs p) {
 
     const uint4* ga = reinterpret_cast<const uint4*>(p.Ap + tm * p.nkt * SLICE_A);
     const uint4* gb = reinterpret_cast<const uint4*>(p.Bp + tn * p.nkt * SLICE_B);
-    constexpr int VA = SLICE_A / 8 / NT;  // 16-byte loads per thread per slice (3)
-    constexpr int VB = SLICE_B / 8 / NT;
-    uint4 ra[VA], rb[VB];
+    static_assert(VA == 3 && VB == 3, "staging macros assume 3 + 3 loads per slice");
+    // LDS slots of this thread's chunks (loop invariant).
+    const int q0 = lds_index(tid), q1 = lds_index(tid + NT), q2 = lds_index(tid + 2 * NT);
+    const int q3 = SLICE_A / 8 + q0, q4 = SLICE_A / 8 + q1, q5 = SLICE_A / 8 + q2;
 
-    auto gload = [&](int64_t kt) {
-        const uint4* a = ga + kt * (SLICE_A / 8);
-        const uint4* b = gb + kt * (SLICE_B / 8);
-#pragma unroll
-        for (int i = 0; i < VA; ++i) ra[i] = a[tid + i * NT];
-#pragma unroll
-        for (int i = 0; i < VB; ++i) rb[i] = b[tid + i * NT];
-    };
-    auto lstore = [&](int buf) {
-        uint4* l = reinterpret_cast<uint4*>(lds[buf]);
-#pragma unroll
-        for (int i = 0; i < VA; ++i) l[tid + i * NT] = ra[i];
-#pragma unroll
-        for (int i = 0; i < VB; ++i) l[SLICE_A / 8 + tid + i * NT] = rb[i];
-    };
-
-    f32x16 hi[2][2], lo[2][2];
+    f32x16 inner[2][2], outer[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) hi[i][j][r] = lo[i][j][r] = 0.f;
-
-    gload(0);
-    lstore(0);
-    __syncthreads();
+            for (int r = 0; r < 16; ++r) inner[i][j][r] = outer[i][j][r] = 0.f;
 
     constexpr int NPLANES = NPROD == 6 ? 3 : 2;
-    // Fragment offsets inside a slice: row (lane % 32), k half (lane / 32).
-    const int frag = (lane & 31) * TK + (lane >> 5) * 8;
-    for (int64_t kt = 0; kt < p.nkt; ++kt) {
-        const int cur = kt & 1;
-        const bool more = kt + 1 < p.nkt;
-        if (more) gload(kt + 1);
-        const __bf16* la = lds[cur];
-        const __bf16* lb = lds[cur] + SLICE_A;
+    const int frow = lane & 31, fh = lane >> 5;
+    auto compute = [&](int buf, int64_t kt) {
+        const uint4* la = lds[buf];
+        const uint4* lb = lds[buf] + SLICE_A / 8;
         bf16x8 a[NPLANES][2], b[NPLANES][2];
 #pragma unroll
         for (int pl = 0; pl < NPLANES; ++pl)
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
-                a[pl][i] = *reinterpret_cast<const bf16x8*>(
-                    la + pl * TM * TK + (wm * 64 + i * 32) * TK + frag);
-                b[pl][i] = *reinterpret_cast<const bf16x8*>(
-                    lb + pl * TN * TK + (wn * 64 + i * 32) * TK + frag);
+                uint4 va = la[pl * 2 * TM + lds_chunk(wm * 64 + i * 32 + frow, fh)];
+                uint4 vb = lb[pl * 2 * TN + lds_chunk(wn * 64 + i * 32 + frow, fh)];
+                a[pl][i] = *reinterpret_cast<const bf16x8*>(&va);
+                b[pl][i] = *reinterpret_cast<const bf16x8*>(&vb);
             }
+        // smallest products first; everything goes into the inner sum
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
                 if (NPROD == 6) {
-                    lo[i][j] = mfma(a[0][i], b[2][j], lo[i][j]);
-                    lo[i][j] = mfma(a[1][i], b[1][j], lo[i][j]);
-                    lo[i][j] = mfma(a[2][i], b[0][j], lo[i][j]);
+                    inner[i][j] = mfma(a[0][i], b[2][j], inner[i][j]);
+                    inner[i][j] = mfma(a[1][i], b[1][j], inner[i][j]);
+                    inner[i][j] = mfma(a[2][i], b[0][j], inner[i][j]);
                 }
-                lo[i][j] = mfma(a[0][i], b[1][j], lo[i][j]);
-                lo[i][j] = mfma(a[1][i], b[0][j], lo[i][j]);
-                hi[i][j] = mfma(a[0][i], b[0][j], hi[i][j]);
+                inner[i][j] = mfma(a[0][i], b[1][j], inner[i][j]);
+                inner[i][j] = mfma(a[1][i], b[0][j], inner[i][j]);
+                inner[i][j] = mfma(a[0][i], b[0][j], inner[i][j]);
             }
-        if (more) lstore(cur ^ 1);
+        if ((kt + 1) % kFoldSlices == 0 || kt + 1 == p.nkt) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    outer[i][j] += inner[i][j];
+                    inner[i][j] = f32x16{};
+                }
+        }
+    };
+
+    const int64_t nkt = p.nkt;
+    uint4 x0, x1, x2, x3, x4, x5;  // even slices
+    uint4 y0, y1, y2, y3, y4, y5;  // odd slices
+    OMB_GLOAD(0, x0, x1, x2, x3, x4, x5);
+    if (nkt > 1) OMB_GLOAD(1, y0, y1, y2, y3, y4, y5);
+    OMB_LSTORE(lds[0], x0, x1, x2, x3, x4, x5);
+    __syncthreads();
+    for (int64_t kt = 0; kt < nkt; kt += 2) {
+        // even slice kt from LDS 0; y* hold kt+1; load kt+2 into x*
+        if (kt + 2 < nkt) OMB_GLOAD(kt + 2, x0, x1, x2, x3, x4, x5);
+        compute(0, kt);
+        if (kt + 1 < nkt) OMB_LSTORE(lds[1], y0, y1, y2, y3, y4, y5);
+        __syncthreads();
+        if (kt + 1 >= nkt) break;
+        // odd slice kt+1 from LDS 1; x* hold kt+2; load kt+3 into y*
+        if (kt + 3 < nkt) OMB_GLOAD(kt + 3, y0, y1, y2, y3, y4, y5);
+        compute(1, kt + 1);
+        if (kt + 2 < nkt) OMB_LSTORE(lds[0], x0, x1, x2, x3, x4, x5);
         __syncthreads();
     }
 
@@ -199,7 +247,7 @@ __global__ __launch_bounds__(NT, 2) void sgemm_bf16x3_kernel(MArgs p) {
                 const int64_t m = m_base + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
                 if (m < p.M) {
                     float* c = p.C + m * p.ldc + n;
-                    float v = p.alpha * (hi[i][j][r] + lo[i][j][r]);
+                    float v = p.alpha * outer[i][j][r];
                     *c = p.beta != 0.f ? v + p.beta * *c : v;
                 }
             }

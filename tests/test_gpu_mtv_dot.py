@@ -51,7 +51,7 @@ def test_vtm_integer_exact_true_transpose(dev, ompblas, order):
 def test_gemv_unaligned_and_ld(dev, ompblas, order):
     big = torch.from_numpy(rand100((403, 517), order)).to(dev) if order == "C" else \
         to_dev(rand100((403, 517), "F"), dev)
-    a = big[3:301, 1:389]                  # offsets break 16-byte alignment
+    a = big[3:301, 1:390]                  # 298 x 389; offsets break 16-byte alignment
     x = torch.from_numpy(rand100((389,))).to(dev)
     y = torch.zeros(298, device=dev)
     ompblas.check(ompblas.sgemv(298, 389, 1.0, a.data_ptr(), a.stride(0), a.stride(1),
