@@ -125,7 +125,8 @@ def main():
     t_pack = timer.run(lambda: lib.omb_split_pack(n, n, A.data_ptr(), n, 1, Ap.data_ptr(), sp),
                        max(3, args.steps // 2))
     kernels = {}
-    for nprod, name in ((6, "sgemm_bf16x3_kernel<6>"), (3, "sgemm_bf16x3_kernel<3>")):
+    for nprod, name in ((6, "sgemm_bf16x3_kernel<6> (pre-packed operands)"),
+                        (3, "sgemm_bf16x3_kernel<3> (pre-packed operands)")):
         fn = lambda: lib.omb_sgemm_packed(n, n, n, 1.0, Ap.data_ptr(), Bp.data_ptr(), 0.0,
                                           C.data_ptr(), n, 1, nprod, sp)
         for _ in range(2):
@@ -134,7 +135,7 @@ def main():
         kernels[name] = {"ms": t_k * 1e3, "mfma_tflops": nprod * flop / t_k / 1e12,
                          "fp32_equiv_tflops": flop / t_k / 1e12,
                          "frac_of_bf16_peak": nprod * flop / t_k / PEAK_MFMA_BF16}
-    t_k6 = kernels["sgemm_bf16x3_kernel<6>"]["ms"] / 1e3
+    t_k6 = kernels["sgemm_bf16x3_kernel<6> (pre-packed operands)"]["ms"] / 1e3
     kernels["split_pack_kernel (8192x8192)"] = {
         "ms": t_pack * 1e3, "gbps": (4 + 6) * n * n / t_pack / 1e9,
         "frac_of_hbm_peak": (4 + 6) * n * n / t_pack / PEAK_HBM}

@@ -160,10 +160,10 @@ extern "C" omb_status omb_sgemm(int64_t M, int64_t N, int64_t K, float alpha, co
     int64_t ars = rsA, acs = csA, brs = rsB, bcs = csB;
     if (!c_row) {
         std::swap(m, n);
-        pa = B;
-        pb = A;
-        ars = csB; acs = rsB;   // A' = B^T : n x k
-        brs = csA; bcs = rsA;   // B' = A^T : k x m
+        pa = B;   // A' = B^T : n x k
+        pb = A;   // B' = A^T : k x m
+        ars = csB; acs = rsB;
+        brs = csA; bcs = rsA;
         std::swap(lda, ldb);
         bool ta = a_row, tb = b_row;
         a_row = !tb;   // B^T is row-major iff B is column-major

@@ -66,7 +66,9 @@ omb_status launch_sgemm_f32(int64_t M, int64_t N, int64_t K, float alpha,
                             const float* B, int64_t ldb, bool b_nc,
                             float beta, float* C, int64_t ldc, hipStream_t s);
 
-// Split-BF16 SGEMM (row-major C after layout folding); operands as strided views.
+// Split-BF16 SGEMM (row-major C after layout folding); operands as strided
+// views: both are split into the packed form in the stream's workspace, then
+// multiplied by the packed kernel.
 omb_status launch_sgemm_split(int64_t M, int64_t N, int64_t K, float alpha,
                               const float* A, int64_t rsA, int64_t csA,
                               const float* B, int64_t rsB, int64_t csB,

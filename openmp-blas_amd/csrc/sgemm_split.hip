@@ -17,76 +17,109 @@
 // Pass 2 (sgemm_bf16x3_kernel): a 256-thread workgroup owns a 128x128 tile of
 // C; four wavefronts each own 64x64 (2x2 blocks of v_mfma_f32_32x32x16_bf16).
 // Slices are prefetched two ahead into registers and staged through LDS
-// (double buffered, bank-swizzled).  All products of 8 slices (128 k) are
+// (double buffered, bank-swizzled).  All products of 16 slices (256 k) are
 // accumulated by the MFMAs into an inner register tile, smallest first, which
 // is then added into an outer tile with FP32 adds: the running sum over K
 // never absorbs MFMA rounding at its own magnitude.
 #include <algorithm>
 
-#include "common.hpp"
+#include "split_common.hpp"
 
 namespace omb {
 namespace {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-
-constexpr int TM = 128, TN = 128, TK = 16;
-constexpr int NT = 256;
-constexpr int kGroupM = 8;
-constexpr int SLICE_A = 3 * TM * TK;  // bf16 elements per A slice (3 planes)
-constexpr int SLICE_B = 3 * TN * TK;
+using namespace split;
 
 // ----------------------------------------------------------------- pass 1 --
-// One thread: 8 consecutive k of one row -> one 16-byte store per plane.
+// One workgroup converts a 128-row x 64-k tile (4 slices) of an operand:
+// coalesced FP32 reads along whichever index is unit-stride, the 3-way split
+// in registers, the packed tile assembled in LDS, then 48 KB of contiguous
+// coalesced stores (a row tile's slices are adjacent in the packed form).
+constexpr int SPLIT_SL = 4;  // slices per workgroup
+
+// Store 4 consecutive k (kk4 .. kk4+3 within the 64-k tile) of one row.
+__device__ inline void put4(__bf16* lds, int row, int kk4, const float (&v)[4]) {
+    typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+    bf16x4 p0, p1, p2;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        __bf16 a, b, c;
+        split3(v[e], a, b, c);
+        p0[e] = a;
+        p1[e] = b;
+        p2[e] = c;
+    }
+    const int sl = kk4 >> 4, k16 = kk4 & 15;
+    __bf16* base = lds + ((sl * 3) * TM + row) * TK + k16;
+    *reinterpret_cast<bf16x4*>(base) = p0;
+    *reinterpret_cast<bf16x4*>(base + TM * TK) = p1;
+    *reinterpret_cast<bf16x4*>(base + 2 * TM * TK) = p2;
+}
+
+__device__ inline float ld(const float* X, int64_t R, int64_t K, int64_t rs, int64_t cs,
+                           int64_t r, int64_t k) {
+    return (r < R && k < K) ? X[r * rs + k * cs] : 0.f;
+}
+
 __global__ __launch_bounds__(NT) void split_pack_kernel(const float* __restrict__ X, int64_t R,
                                                         int64_t K, int64_t rs, int64_t cs,
-                                                        int64_t Rp, int64_t Kp, int br,
-                                                        int k_major, __bf16* __restrict__ out) {
-    const int64_t groups = Kp >> 3;
-    const int64_t total = Rp * groups;
-    const int64_t nkt = Kp / TK;
-    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < total;
-         i += (int64_t)gridDim.x * NT) {
-        int64_t r, g;
-        if (k_major) {
-            r = i / groups;
-            g = i - r * groups;
-        } else {
-            g = i / Rp;
-            r = i - g * Rp;
-        }
-        const int64_t k0 = g << 3;
-        float v[8];
-        if (r < R && cs == 1 && k0 + 8 <= K &&
-            ((reinterpret_cast<uintptr_t>(X + r * rs + k0) & 15) == 0)) {
-            const float4* p = reinterpret_cast<const float4*>(X + r * rs + k0);
-            float4 a = p[0], b = p[1];
-            v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-            v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
-        } else {
-#pragma unroll
-            for (int j = 0; j < 8; ++j)
-                v[j] = (r < R && k0 + j < K) ? X[r * rs + (k0 + j) * cs] : 0.f;
-        }
-        bf16x8 h0, h1, h2;
+                                                        int64_t nkt, int k_major,
+                                                        __bf16* __restrict__ out) {
+    __shared__ __attribute__((aligned(16))) __bf16 lds[SPLIT_SL * 3 * TM * TK];  // 48 KB
+    const int tid = threadIdx.x;
+    const int64_t rt = blockIdx.x, kt0 = (int64_t)blockIdx.y * SPLIT_SL;
+    const int64_t r0 = rt * TM, k0 = kt0 * TK;
+    // 128-bit loads need a unit stride along the loaded index and 16-byte
+    // aligned rows/columns; any other strides take the element path.
+    const bool vec = ((reinterpret_cast<uintptr_t>(X) & 15) == 0) &&
+                     (k_major ? (cs == 1 && rs % 4 == 0) : (rs == 1 && cs % 4 == 0));
+    if (k_major) {
+        // 128 rows x 16 float4 along k: thread -> (row, k4), 8 per thread
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            __bf16 t0 = (__bf16)v[j];
-            float r1 = v[j] - (float)t0;
-            __bf16 t1 = (__bf16)r1;
-            float r2 = r1 - (float)t1;
-            h0[j] = t0;
-            h1[j] = t1;
-            h2[j] = (__bf16)r2;
+            const int f = tid + NT * j, row = f >> 4, kk4 = (f & 15) * 4;
+            const int64_t r = r0 + row, k = k0 + kk4;
+            float v[4];
+            if (vec && r < R && k + 3 < K) {
+                float4 t = *reinterpret_cast<const float4*>(X + r * rs + k);
+                v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) v[e] = ld(X, R, K, rs, cs, r, k + e);
+            }
+            put4(lds, row, kk4, v);
         }
-        const int64_t rt = r / br, rr = r - rt * br;
-        const int64_t kt = k0 / TK, kk = k0 - kt * TK;
-        __bf16* base = out + ((rt * nkt + kt) * 3) * (int64_t)(br * TK) + rr * TK + kk;
-        *reinterpret_cast<bf16x8*>(base) = h0;
-        *reinterpret_cast<bf16x8*>(base + br * TK) = h1;
-        *reinterpret_cast<bf16x8*>(base + 2 * br * TK) = h2;
+    } else {
+        // 64 k-columns x 32 float4 along rows: thread -> (4 columns, 4 rows)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int f = tid + NT * j, kq = f >> 5, row4 = (f & 31) * 4;
+            const int64_t r = r0 + row4;
+            float m[4][4];  // [k][row]
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int64_t k = k0 + kq * 4 + c;
+                if (vec && k < K && r + 3 < R) {
+                    float4 t = *reinterpret_cast<const float4*>(X + k * cs + r);
+                    m[c][0] = t.x; m[c][1] = t.y; m[c][2] = t.z; m[c][3] = t.w;
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) m[c][e] = ld(X, R, K, rs, cs, r + e, k);
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float v[4] = {m[0][e], m[1][e], m[2][e], m[3][e]};
+                put4(lds, row4 + e, kq * 4, v);
+            }
+        }
     }
+    __syncthreads();
+    const int nsl = (int)(nkt - kt0 < SPLIT_SL ? nkt - kt0 : SPLIT_SL);
+    const uint4* src = reinterpret_cast<const uint4*>(lds);
+    uint4* dst = reinterpret_cast<uint4*>(out + (rt * nkt + kt0) * (int64_t)SLICE_A);
+    const int n16 = nsl * SLICE_A / 8;
+    for (int i = tid; i < n16; i += NT) dst[i] = src[i];
 }
 
 // ----------------------------------------------------------------- pass 2 --
@@ -101,16 +134,6 @@ struct MArgs {
     int64_t tiles_m, tiles_n;
 };
 
-__device__ inline f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c) {
-    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
-}
-
-// LDS position of 16-byte chunk h (0/1) of slice row r: the chunk index is
-// flipped by bits 2 and 3 of the row so that the 8 or 16 lanes a ds_read_b128
-// serves per cycle (rows r..r+15, same h) hit distinct banks.
-__device__ inline int lds_chunk(int r, int h) { return (r << 1) | (h ^ ((r >> 2) & 1) ^ ((r >> 3) & 1)); }
-
-constexpr int kFoldSlices = 8;  // inner accumulation spans 8 slices = 128 k
 constexpr int VA = SLICE_A / 8 / NT;  // 16-byte loads per thread per slice (3)
 constexpr int VB = SLICE_B / 8 / NT;
 constexpr int V = VA + VB;
@@ -168,53 +191,17 @@ __global__ __launch_bounds__(NT, 2) void sgemm_bf16x3_kernel(MArgs p) {
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) inner[i][j][r] = outer[i][j][r] = 0.f;
-
-    constexpr int NPLANES = NPROD == 6 ? 3 : 2;
+        for (int j = 0; j < 2; ++j) inner[i][j] = outer[i][j] = f32x16{};
     const int frow = lane & 31, fh = lane >> 5;
     auto compute = [&](int buf, int64_t kt) {
-        const uint4* la = lds[buf];
-        const uint4* lb = lds[buf] + SLICE_A / 8;
-        bf16x8 a[NPLANES][2], b[NPLANES][2];
-#pragma unroll
-        for (int pl = 0; pl < NPLANES; ++pl)
-#pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                uint4 va = la[pl * 2 * TM + lds_chunk(wm * 64 + i * 32 + frow, fh)];
-                uint4 vb = lb[pl * 2 * TN + lds_chunk(wn * 64 + i * 32 + frow, fh)];
-                a[pl][i] = *reinterpret_cast<const bf16x8*>(&va);
-                b[pl][i] = *reinterpret_cast<const bf16x8*>(&vb);
-            }
-        // smallest products first; everything goes into the inner sum
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                if (NPROD == 6) {
-                    inner[i][j] = mfma(a[0][i], b[2][j], inner[i][j]);
-                    inner[i][j] = mfma(a[1][i], b[1][j], inner[i][j]);
-                    inner[i][j] = mfma(a[2][i], b[0][j], inner[i][j]);
-                }
-                inner[i][j] = mfma(a[0][i], b[1][j], inner[i][j]);
-                inner[i][j] = mfma(a[1][i], b[0][j], inner[i][j]);
-                inner[i][j] = mfma(a[0][i], b[0][j], inner[i][j]);
-            }
-        if ((kt + 1) % kFoldSlices == 0 || kt + 1 == p.nkt) {
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    outer[i][j] += inner[i][j];
-                    inner[i][j] = f32x16{};
-                }
-        }
+        mfma_slice<NPROD>(lds[buf], wm, wn, frow, fh, inner);
     };
-
+    auto fold_at = [&](int64_t kt) {
+        if ((kt + 1) % kFoldSlices == 0 || kt + 1 == p.nkt) fold(inner, outer);
+    };
     const int64_t nkt = p.nkt;
-    uint4 x0, x1, x2, x3, x4, x5;  // even slices
-    uint4 y0, y1, y2, y3, y4, y5;  // odd slices
+    uint4 x0, x1, x2, x3, x4, x5;  // even slices in flight
+    uint4 y0, y1, y2, y3, y4, y5;  // odd slices in flight
     OMB_GLOAD(0, x0, x1, x2, x3, x4, x5);
     if (nkt > 1) OMB_GLOAD(1, y0, y1, y2, y3, y4, y5);
     OMB_LSTORE(lds[0], x0, x1, x2, x3, x4, x5);
@@ -223,35 +210,20 @@ __global__ __launch_bounds__(NT, 2) void sgemm_bf16x3_kernel(MArgs p) {
         // even slice kt from LDS 0; y* hold kt+1; load kt+2 into x*
         if (kt + 2 < nkt) OMB_GLOAD(kt + 2, x0, x1, x2, x3, x4, x5);
         compute(0, kt);
+        fold_at(kt);
         if (kt + 1 < nkt) OMB_LSTORE(lds[1], y0, y1, y2, y3, y4, y5);
         __syncthreads();
         if (kt + 1 >= nkt) break;
         // odd slice kt+1 from LDS 1; x* hold kt+2; load kt+3 into y*
         if (kt + 3 < nkt) OMB_GLOAD(kt + 3, y0, y1, y2, y3, y4, y5);
         compute(1, kt + 1);
+        fold_at(kt + 1);
         if (kt + 2 < nkt) OMB_LSTORE(lds[0], x0, x1, x2, x3, x4, x5);
         __syncthreads();
     }
 
-    // Epilogue.  32x32 accumulator layout: lane l holds column l % 32, value r
-    // holds row 8*(r/4) + 4*(l/32) + r%4.
-    const int64_t m_base = tm * TM + wm * 64, n_base = tn * TN + wn * 64;
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int64_t n = n_base + j * 32 + (lane & 31);
-            if (n >= p.N) continue;
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int64_t m = m_base + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                if (m < p.M) {
-                    float* c = p.C + m * p.ldc + n;
-                    float v = p.alpha * outer[i][j][r];
-                    *c = p.beta != 0.f ? v + p.beta * *c : v;
-                }
-            }
-        }
+    store_tile(outer, tm * TM + wm * 64, tn * TN + wn * 64, lane, p.C, p.ldc, p.M, p.N,
+               p.alpha, p.beta);
 }
 
 }  // namespace
@@ -264,12 +236,15 @@ int64_t split_packed_elems(int64_t R, int64_t K) {
 omb_status launch_split_pack(int64_t R, int64_t K, const float* X, int64_t rs, int64_t cs,
                              __bf16* out, hipStream_t s) {
     if (R <= 0 || K <= 0) return OMB_OK;
-    const int64_t Rp = round_up(R, TM), Kp = round_up(K, TK);
-    const int64_t total = Rp * (Kp >> 3);
-    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(total, NT), 8192));
-    const int k_major = (cs == 1) ? 1 : 0;
-    hipLaunchKernelGGL(split_pack_kernel, dim3((unsigned)blocks), dim3(NT), 0, s, X, R, K, rs, cs,
-                       Rp, Kp, (int)TM, k_major, out);
+    const int64_t rtiles = ceil_div(R, TM), nkt = ceil_div(K, TK);
+    const int64_t kgroups = ceil_div(nkt, SPLIT_SL);
+    if (rtiles > 0x7fffffffLL || kgroups > 65535) {
+        set_error("split pack: operand too large for one launch");
+        return OMB_EUNSUPPORTED;
+    }
+    const int k_major = cs == 1 ? 1 : 0;
+    hipLaunchKernelGGL(split_pack_kernel, dim3((unsigned)rtiles, (unsigned)kgroups), dim3(NT), 0,
+                       s, X, R, K, rs, cs, nkt, k_major, out);
     OMB_HIP_TRY(hipGetLastError());
     return OMB_OK;
 }

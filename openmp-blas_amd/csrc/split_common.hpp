@@ -1,0 +1,134 @@
+// Shared pieces of the split-BF16 GEMM kernels: tile sizes, the MFMA, the
+// swizzled LDS slice layout and the exact 3-way FP32 -> BF16 split.
+#pragma once
+
+#include "common.hpp"
+
+namespace omb {
+namespace split {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int TM = 128, TN = 128, TK = 16;  // C tile and k slice of one workgroup
+constexpr int NT = 256;                      // threads per workgroup (4 wavefronts)
+constexpr int kGroupM = 8;                   // tile rows per L2 group
+constexpr int SLICE_A = 3 * TM * TK;         // bf16 elements per A slice (3 planes)
+constexpr int SLICE_B = 3 * TN * TK;
+constexpr int kFoldSlices = 16;              // inner MFMA accumulation spans 256 k
+
+__device__ inline f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+// LDS position (in 16-byte chunks, within one plane) of chunk h (0/1) of
+// slice row r: the chunk index is flipped by bits 2 and 3 of the row so that
+// the 8 or 16 lanes a ds_read_b128 serves per cycle (rows r..r+15, same h)
+// hit distinct banks.
+__device__ inline int lds_chunk(int r, int h) {
+    return (r << 1) | (h ^ ((r >> 2) & 1) ^ ((r >> 3) & 1));
+}
+
+// x = h0 + h1 + h2 exactly (each subtraction is exact in FP32).
+__device__ inline void split3(float x, __bf16& h0, __bf16& h1, __bf16& h2) {
+    h0 = (__bf16)x;
+    const float r1 = x - (float)h0;
+    h1 = (__bf16)r1;
+    h2 = (__bf16)(r1 - (float)h1);
+}
+
+// Split 8 consecutive-k values into the three 16-byte plane chunks.
+__device__ inline void split8(float4 lo, float4 hi, uint4& p0, uint4& p1, uint4& p2) {
+    const float v[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+    bf16x8 a, b, c;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        __bf16 t0, t1, t2;
+        split3(v[j], t0, t1, t2);
+        a[j] = t0;
+        b[j] = t1;
+        c[j] = t2;
+    }
+    p0 = *reinterpret_cast<const uint4*>(&a);
+    p1 = *reinterpret_cast<const uint4*>(&b);
+    p2 = *reinterpret_cast<const uint4*>(&c);
+}
+
+}  // namespace split
+}  // namespace omb
+
+namespace omb {
+namespace split {
+
+// MFMAs of one k slice for one wavefront (64x64 of C as 2x2 blocks of 32x32):
+// fragments from the swizzled LDS slice, smallest products first, all into
+// the inner accumulators.
+template <int NPROD>
+__device__ __forceinline__ void mfma_slice(const uint4* slice, int wm, int wn, int frow, int fh,
+                                           f32x16 (&inner)[2][2]) {
+    constexpr int NPLANES = NPROD == 6 ? 3 : 2;
+    const uint4* la = slice;
+    const uint4* lb = slice + SLICE_A / 8;
+    bf16x8 a[NPLANES][2], b[NPLANES][2];
+#pragma unroll
+    for (int pl = 0; pl < NPLANES; ++pl)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            uint4 va = la[pl * 2 * TM + lds_chunk(wm * 64 + i * 32 + frow, fh)];
+            uint4 vb = lb[pl * 2 * TN + lds_chunk(wn * 64 + i * 32 + frow, fh)];
+            a[pl][i] = *reinterpret_cast<const bf16x8*>(&va);
+            b[pl][i] = *reinterpret_cast<const bf16x8*>(&vb);
+        }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            if (NPROD == 6) {
+                inner[i][j] = mfma(a[0][i], b[2][j], inner[i][j]);
+                inner[i][j] = mfma(a[1][i], b[1][j], inner[i][j]);
+                inner[i][j] = mfma(a[2][i], b[0][j], inner[i][j]);
+            }
+            inner[i][j] = mfma(a[0][i], b[1][j], inner[i][j]);
+            inner[i][j] = mfma(a[1][i], b[0][j], inner[i][j]);
+            inner[i][j] = mfma(a[0][i], b[0][j], inner[i][j]);
+        }
+}
+
+// outer += inner; inner = 0 (FP32 adds: the running sum over K is rounded
+// by the vector ALUs, not inside the MFMA accumulation).
+__device__ __forceinline__ void fold(f32x16 (&inner)[2][2], f32x16 (&outer)[2][2]) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            outer[i][j] += inner[i][j];
+            inner[i][j] = f32x16{};
+        }
+}
+
+// C = alpha * acc + beta * C for the wavefront's 64x64 block (row-major C).
+// 32x32 accumulator layout: lane l holds column l % 32, value r holds row
+// 8*(r/4) + 4*(l/32) + r%4.
+__device__ __forceinline__ void store_tile(const f32x16 (&acc)[2][2], int64_t m_base,
+                                           int64_t n_base, int lane, float* C, int64_t ldc,
+                                           int64_t M, int64_t N, float alpha, float beta) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int64_t n = n_base + j * 32 + (lane & 31);
+            if (n >= N) continue;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int64_t m = m_base + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (m < M) {
+                    float* c = C + m * ldc + n;
+                    const float v = alpha * acc[i][j][r];
+                    *c = beta != 0.f ? v + beta * *c : v;
+                }
+            }
+        }
+}
+
+}  // namespace split
+}  // namespace omb

@@ -213,6 +213,19 @@ def rng_pow2(n: int, seed: int, stream: int) -> np.ndarray:
     return (s * np.exp2(u)).astype(np.float32)
 
 
+# ------------------------------------------------------ launch pre-checks --
+def assert_view_in_bounds(t, rows: int, cols: int, rs: int, cs: int) -> None:
+    """Before a kernel reads or writes the view (i, j) -> base[i*rs + j*cs],
+    check on the host that every element lies inside t's storage."""
+    if rows == 0 or cols == 0:
+        return
+    storage = t.untyped_storage().nbytes() // t.element_size()
+    first = t.storage_offset()
+    last = first + (rows - 1) * rs + (cols - 1) * cs
+    assert rs >= 1 and cs >= 1 and last < storage, (
+        f"view {rows}x{cols} strides ({rs},{cs}) offset {first} exceeds storage {storage}")
+
+
 # ---------------------------------------------------------------- errors --
 def rel_errors(c, truth) -> tuple:
     """(max-relative, Frobenius-relative) of c against the FP64 truth."""

@@ -28,6 +28,9 @@ def to_dev(a: np.ndarray, dev) -> torch.Tensor:
 def run_mtm(ompblas, c, a, b, mode, alpha=1.0, beta=1.0):
     M, K = a.shape
     N = b.shape[1]
+    helpers.assert_view_in_bounds(a, M, K, a.stride(0), a.stride(1))
+    helpers.assert_view_in_bounds(b, K, N, b.stride(0), b.stride(1))
+    helpers.assert_view_in_bounds(c, M, N, c.stride(0), c.stride(1))
     st = ompblas.sgemm(M, N, K, alpha, a.data_ptr(), a.stride(0), a.stride(1), b.data_ptr(),
                        b.stride(0), b.stride(1), beta, c.data_ptr(), c.stride(0), c.stride(1),
                        mode)
@@ -173,3 +176,34 @@ def test_mtm_8192_slab_error_vs_reference(dev, ompblas, layout):
         got = rel_errors(tc[rows].cpu().numpy(), truth)
         print(layout, mode, "gpu", got, "oracle", ref)
         assert got[0] <= 2 * ref[0] and got[1] <= 2 * ref[1], (mode, got, ref)
+
+
+@pytest.mark.parametrize("nprod", [6, 3])
+@pytest.mark.parametrize("shape", [(256, 384, 512), (130, 77, 1000), (1, 129, 17)])
+def test_packed_split_path_matches_one_call(dev, ompblas, shape, nprod):
+    """omb_split_pack + omb_sgemm_packed (operands split once, reusable) gives
+    bitwise the result of omb_sgemm in the same split mode, for both layouts
+    of C."""
+    M, N, K = shape
+    lib = ompblas.lib()
+    a = torch.empty((M, K), device=dev)
+    b = torch.empty((N, K), device=dev).t()       # column-major K x N
+    ompblas.fill_uniform(a, 17, 0)
+    ompblas.fill_uniform(b.t(), 17, 1)
+    pa = torch.empty(lib.omb_split_pack_bytes(M, K), dtype=torch.uint8, device=dev)
+    pb = torch.empty(lib.omb_split_pack_bytes(N, K), dtype=torch.uint8, device=dev)
+    # B^T (N x K) of the column-major B (strides (1, K)) has strides (K, 1).
+    bt_rs, bt_cs = b.stride(1), b.stride(0)
+    helpers.assert_view_in_bounds(a, M, K, K, 1)
+    helpers.assert_view_in_bounds(b, N, K, bt_rs, bt_cs)
+    ompblas.check(lib.omb_split_pack(M, K, a.data_ptr(), K, 1, pa.data_ptr(), None))
+    ompblas.check(lib.omb_split_pack(N, K, b.data_ptr(), bt_rs, bt_cs, pb.data_ptr(), None))
+    mode = 1 if nprod == 6 else 2
+    for c_order in ("C", "F"):
+        want = to_dev(np.zeros((M, N), np.float32, order=c_order), dev)
+        run_mtm(ompblas, want, a, b, mode)
+        got = to_dev(np.zeros((M, N), np.float32, order=c_order), dev)
+        ompblas.check(lib.omb_sgemm_packed(M, N, K, 1.0, pa.data_ptr(), pb.data_ptr(), 0.0,
+                                           got.data_ptr(), got.stride(0), got.stride(1), nprod,
+                                           None))
+        assert torch.equal(got, want), (shape, nprod, c_order)

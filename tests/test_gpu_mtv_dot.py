@@ -54,6 +54,7 @@ def test_gemv_unaligned_and_ld(dev, ompblas, order):
     a = big[3:301, 1:390]                  # 298 x 389; offsets break 16-byte alignment
     x = torch.from_numpy(rand100((389,))).to(dev)
     y = torch.zeros(298, device=dev)
+    helpers.assert_view_in_bounds(a, 298, 389, a.stride(0), a.stride(1))
     ompblas.check(ompblas.sgemv(298, 389, 1.0, a.data_ptr(), a.stride(0), a.stride(1),
                                 x.data_ptr(), 0.0, y.data_ptr()))
     want = a.cpu().numpy().astype(np.int64) @ x.cpu().numpy().astype(np.int64)
@@ -95,6 +96,7 @@ def test_gemv_deterministic_and_beta(dev, ompblas):
     ompblas.fill_uniform(x, 8, 1)
     y0 = torch.empty(M, device=dev)
     ompblas.fill_uniform(y0, 8, 2)
+    helpers.assert_view_in_bounds(a, M, N, a.stride(0), a.stride(1))
     outs = []
     for _ in range(3):
         y = y0.clone()
