@@ -327,3 +327,52 @@ def rel_errors(c, c64, stream=None):
     """(max-relative, Frobenius-relative) error of c against c64."""
     dmax, rmax, d2, r2 = errnorm(c, c64, stream)
     return dmax / rmax if rmax > 0 else dmax, (d2 / r2) ** 0.5 if r2 > 0 else d2 ** 0.5
+
+
+# --------------------------------------------------------- grouped plans --
+class GroupPlan:
+    """A captured grouped launch (omb_group_plan_*): GEMMs given as tuples
+    (M, N, K, A, lda, B, ldb, C, ldc, alpha, beta) and GEMVs as
+    (M, N, A, lda, x, y, alpha, beta) of device pointers (row-major)."""
+
+    def __init__(self, gemms, gemvs, mode: int = PLAIN_F32, stream=None):
+        g = (GemmDesc * max(1, len(gemms)))(*[GemmDesc(*t) for t in gemms])
+        v = (GemvDesc * max(1, len(gemvs)))(*[GemvDesc(*t) for t in gemvs])
+        self._h = ctypes.c_void_p()
+        check(lib().omb_group_plan_create(g, len(gemms), v, len(gemvs), mode, _stream(stream),
+                                          ctypes.byref(self._h)))
+
+    def launch(self, stream=None) -> None:
+        check(lib().omb_group_plan_launch(self._h, _stream(stream)))
+
+    def launch_direct(self, stream=None) -> None:
+        check(lib().omb_group_plan_launch_direct(self._h, _stream(stream)))
+
+    def describe(self) -> str:
+        buf = ctypes.create_string_buffer(4096)
+        check(lib().omb_group_plan_describe(self._h, buf, len(buf)))
+        return buf.value.decode()
+
+    def close(self) -> None:
+        if self._h:
+            lib().omb_group_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune_group(gemms, mode: int = PLAIN_F32, table_path: Optional[str] = None,
+                   stream=None) -> None:
+    """Time the tile classes on each (log2 M, log2 N, log2 K) bucket of
+    `gemms` (tuples as for GroupPlan) and keep the fastest per bucket."""
+    g = (GemmDesc * max(1, len(gemms)))(*[GemmDesc(*t) for t in gemms])
+    check(lib().omb_autotune_group(g, len(gemms), mode,
+                                   table_path.encode() if table_path else None, _stream(stream)))
+
+
+def autotune_load(table_path: str) -> None:
+    check(lib().omb_autotune_load(table_path.encode()))

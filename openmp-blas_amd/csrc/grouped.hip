@@ -1,0 +1,751 @@
+// Grouped execution: thousands of independent GEMMs and GEMVs in ONE
+// persistent kernel launch, captured in a hipGraph.  (The reference has no
+// batched API; a caller loops over mtm_helper / mtv_helper.)
+//
+// Work list: every GEMM is cut into tiles of the class the autotuner chose
+// for its (log2 M, log2 N, log2 K) bucket, every GEMV into 64-row chunks;
+// items are sorted by estimated cost (longest first).  A fixed grid of
+// workgroups (CUs x occupancy) pulls items from an atomic counter; the last
+// workgroup to finish resets the counters (atomics only: no cache-flushing
+// fences, no memset node in the graph).
+//
+// Every tile class keeps the per-element arithmetic of the standalone kernels
+// (plain: sequential k FMAs folded every 128 k; split-BF16: the same MFMA
+// slices, product order and 256-k folds; GEMV: the same lane order and
+// butterfly), so a grouped result is bitwise the one omb_sgemm / omb_sgemv
+// gives for that instance alone -- whatever else is in the batch.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "gemv_row.hpp"
+#include "split_common.hpp"
+
+namespace omb {
+namespace {
+
+struct DGemm {
+    int64_t M, N, K;
+    const float* A;
+    int64_t lda;
+    const float* B;
+    int64_t ldb;
+    float* C;
+    int64_t ldc;
+    float alpha, beta;
+};
+
+struct DGemv {
+    int64_t M, N;
+    const float* A;
+    int64_t lda;
+    const float* x;
+    float* y;
+    float alpha, beta;
+    int vec;
+};
+
+enum Cls : int { C32 = 0, C64 = 1, C128x64 = 2, CSPLIT = 3, CGEMV = 4, NCLS = 5 };
+const char* const kClsName[NCLS] = {"f32_32x32", "f32_64x64", "f32_128x64", "bf16x3_64x64",
+                                    "gemv_64rows"};
+constexpr int kTileM[NCLS] = {32, 64, 128, 64, 64};
+constexpr int kTileN[NCLS] = {32, 64, 64, 64, 1};
+
+struct Item {
+    int cls, inst, tm, tn;
+};
+
+struct PArgs {
+    const Item* items;
+    int nitems;
+    const DGemm* gemms;
+    const DGemv* gemvs;
+    unsigned* counters;  // [0] next item, [1] workgroups done
+    int nprod;
+};
+
+constexpr int NT = 256;
+constexpr int BK = 16;
+constexpr int kFold = 8;  // 8 slices of 16 = 128 k per inner block (as sgemm_f32)
+
+// ------------------------------------------------------------ plain tile --
+// BM x BN tile of C = alpha*A*B + beta*C (all row-major); 256 threads on a
+// (BM/TMR) x (BN/TNR) grid, thread (ty, tx) owns rows ty + i*H, columns
+// tx + j*W.  One LDS slice, the next one prefetched into registers.
+template <int BM, int BN, int TMR, int TNR>
+__device__ __forceinline__ void tile_f32(const DGemm& g, int tm, int tn, float* smem) {
+    constexpr int H = BM / TMR, W = BN / TNR;
+    static_assert(H * W == NT, "thread grid must cover the workgroup");
+    constexpr int LA = BM * BK / NT, LB = BN * BK / NT;
+    constexpr int PA = BM + 1, PB = BN + 1;  // padded LDS rows (conflict-free stores)
+    float* As = smem;            // [BK][PA]
+    float* Bs = smem + BK * PA;  // [BK][PB]
+    const int tid = threadIdx.x, tx = tid % W, ty = tid / W;
+    const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
+    const int64_t M = g.M, N = g.N, K = g.K;
+
+    float ra[LA], rb[LB];
+    auto load = [&](int64_t k0) {
+#pragma unroll
+        for (int i = 0; i < LA; ++i) {
+            const int e = tid + i * NT, r = e / BK, k = e % BK;
+            const int64_t m = m0 + r, kk = k0 + k;
+            ra[i] = (m < M && kk < K) ? g.A[m * g.lda + kk] : 0.f;
+        }
+#pragma unroll
+        for (int i = 0; i < LB; ++i) {
+            const int e = tid + i * NT, k = e / BN, c = e % BN;
+            const int64_t n = n0 + c, kk = k0 + k;
+            rb[i] = (kk < K && n < N) ? g.B[kk * g.ldb + n] : 0.f;
+        }
+    };
+
+    float inner[TMR][TNR], outer[TMR][TNR];
+#pragma unroll
+    for (int i = 0; i < TMR; ++i)
+#pragma unroll
+        for (int j = 0; j < TNR; ++j) inner[i][j] = outer[i][j] = 0.f;
+
+    const int64_t nk = ceil_div(K, BK);
+    load(0);
+    for (int64_t kt = 0; kt < nk; ++kt) {
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < LA; ++i) {
+            const int e = tid + i * NT;
+            As[(e % BK) * PA + e / BK] = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < LB; ++i) {
+            const int e = tid + i * NT;
+            Bs[(e / BN) * PB + e % BN] = rb[i];
+        }
+        __syncthreads();
+        if (kt + 1 < nk) load((kt + 1) * BK);
+#pragma unroll
+        for (int k = 0; k < BK; ++k) {
+            float a[TMR], b[TNR];
+#pragma unroll
+            for (int i = 0; i < TMR; ++i) a[i] = As[k * PA + ty + i * H];
+#pragma unroll
+            for (int j = 0; j < TNR; ++j) b[j] = Bs[k * PB + tx + j * W];
+#pragma unroll
+            for (int i = 0; i < TMR; ++i)
+#pragma unroll
+                for (int j = 0; j < TNR; ++j) inner[i][j] = fmaf(a[i], b[j], inner[i][j]);
+        }
+        if ((kt + 1) % kFold == 0 || kt + 1 == nk) {
+#pragma unroll
+            for (int i = 0; i < TMR; ++i)
+#pragma unroll
+                for (int j = 0; j < TNR; ++j) {
+                    outer[i][j] += inner[i][j];
+                    inner[i][j] = 0.f;
+                }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < TMR; ++i) {
+        const int64_t m = m0 + ty + i * H;
+        if (m >= M) continue;
+#pragma unroll
+        for (int j = 0; j < TNR; ++j) {
+            const int64_t n = n0 + tx + j * W;
+            if (n < N) {
+                float* c = g.C + m * g.ldc + n;
+                const float v = g.alpha * outer[i][j];
+                *c = g.beta != 0.f ? v + g.beta * *c : v;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------ split tile --
+// 64x64 tile on the MFMA cores: each 16-k slice of A (row-major) and B^T
+// (B row-major) is read as FP32, split into the three BF16 planes and
+// written into the swizzled LDS slice; 4 wavefronts own 32x32 each.
+template <int NPROD>
+__device__ __forceinline__ void tile_split(const DGemm& g, int tm, int tn, uint4* smem) {
+    using namespace split;
+    constexpr int P = 64;  // rows per plane for A and for B^T
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int frow = lane & 31, fh = lane >> 5;
+    const int64_t m0 = (int64_t)tm * P, n0 = (int64_t)tn * P;
+    const int64_t M = g.M, N = g.N, K = g.K;
+    const bool vec = ((reinterpret_cast<uintptr_t>(g.A) & 15) == 0) && (g.lda % 4 == 0);
+    // loader roles: A row ra, k quarter qa; B^T row rb (= column of B), k quarter qb
+    const int ra = tid >> 2, qa = tid & 3;
+    const int rb = tid & 63, qb = tid >> 6;
+    uint2* l2 = reinterpret_cast<uint2*>(smem);
+
+    f32x16 inner[1][1], outer[1][1];
+    inner[0][0] = outer[0][0] = f32x16{};
+    const int64_t nkt = ceil_div(K, TK);
+    for (int64_t kt = 0; kt < nkt; ++kt) {
+        const int64_t k0 = kt * TK;
+        float va[4], vb[4];
+        {
+            const int64_t m = m0 + ra, k = k0 + 4 * qa;
+            if (vec && m < M && k + 3 < K) {
+                const float4 t = *reinterpret_cast<const float4*>(g.A + m * g.lda + k);
+                va[0] = t.x; va[1] = t.y; va[2] = t.z; va[3] = t.w;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) va[e] = (m < M && k + e < K) ? g.A[m * g.lda + k + e] : 0.f;
+            }
+        }
+        {
+            const int64_t n = n0 + rb, k = k0 + 4 * qb;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) vb[e] = (n < N && k + e < K) ? g.B[(k + e) * g.ldb + n] : 0.f;
+        }
+        __syncthreads();  // previous slice consumed
+        {
+            typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+            bf16x4 pa[3], pb[3];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                __bf16 t0, t1, t2;
+                split3(va[e], t0, t1, t2);
+                pa[0][e] = t0; pa[1][e] = t1; pa[2][e] = t2;
+                split3(vb[e], t0, t1, t2);
+                pb[0][e] = t0; pb[1][e] = t1; pb[2][e] = t2;
+            }
+#pragma unroll
+            for (int pl = 0; pl < 3; ++pl) {
+                l2[2 * (pl * 2 * P + lds_chunk(ra, qa >> 1)) + (qa & 1)] =
+                    *reinterpret_cast<const uint2*>(&pa[pl]);
+                l2[2 * (3 * 2 * P + pl * 2 * P + lds_chunk(rb, qb >> 1)) + (qb & 1)] =
+                    *reinterpret_cast<const uint2*>(&pb[pl]);
+            }
+        }
+        __syncthreads();
+        mfma_slice<NPROD, 1, 1, P, P>(smem, wm * 32, wn * 32, frow, fh, inner);
+        if ((kt + 1) % kFoldSlices == 0 || kt + 1 == nkt) fold(inner, outer);
+    }
+    store_tile(outer, m0 + wm * 32, n0 + wn * 32, lane, g.C, g.ldc, M, N, g.alpha, g.beta);
+}
+
+// ------------------------------------------------------------- GEMV item --
+__device__ __forceinline__ void gemv_item(const DGemv& v, int chunk) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // wavefront w takes rows chunk*64 + 16w .. +15, four at a time
+#pragma unroll 1
+    for (int q = 0; q < 4; ++q) {
+        const int64_t r0 = (int64_t)chunk * 64 + wave * 16 + q * kGemvRows;
+        if (r0 >= v.M) break;
+        const int nr = (int)(v.M - r0 < kGemvRows ? v.M - r0 : kGemvRows);
+        if (v.vec) gemv_rows_wave<true, 2>(r0, nr, v.N, v.alpha, v.A, v.lda, v.x, v.beta, v.y, lane);
+        else gemv_rows_wave<false, 2>(r0, nr, v.N, v.alpha, v.A, v.lda, v.x, v.beta, v.y, lane);
+    }
+}
+
+// ------------------------------------------------------- persistent kernel --
+constexpr int kSmemFloats = BK * (128 + 1) + BK * (64 + 1) + 16;  // 128x64 tile; split uses 12 KB
+
+template <bool BIG>
+__global__ __launch_bounds__(NT, 2) void grouped_kernel(PArgs p) {
+    __shared__ __attribute__((aligned(16))) float smem[kSmemFloats];
+    __shared__ int s_item;
+    for (;;) {
+        __syncthreads();  // the previous item is done with LDS and s_item
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(&p.counters[0], 1u);
+        __syncthreads();
+        // s_item is the same for the whole workgroup: make that visible to the
+        // compiler so the descriptors live in scalar registers.
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
+        if (it >= p.nitems) break;
+        Item w = p.items[it];
+        w.cls = __builtin_amdgcn_readfirstlane(w.cls);
+        w.inst = __builtin_amdgcn_readfirstlane(w.inst);
+        w.tm = __builtin_amdgcn_readfirstlane(w.tm);
+        w.tn = __builtin_amdgcn_readfirstlane(w.tn);
+        switch (w.cls) {
+            case C32: tile_f32<32, 32, 2, 2>(p.gemms[w.inst], w.tm, w.tn, smem); break;
+            case C64: tile_f32<64, 64, 4, 4>(p.gemms[w.inst], w.tm, w.tn, smem); break;
+            case C128x64:
+                if (BIG) tile_f32<128, 64, 8, 4>(p.gemms[w.inst], w.tm, w.tn, smem);
+                break;
+            case CSPLIT:
+                if (p.nprod == 3)
+                    tile_split<3>(p.gemms[w.inst], w.tm, w.tn, reinterpret_cast<uint4*>(smem));
+                else
+                    tile_split<6>(p.gemms[w.inst], w.tm, w.tn, reinterpret_cast<uint4*>(smem));
+                break;
+            case CGEMV: gemv_item(p.gemvs[w.inst], w.tm); break;
+            default: break;
+        }
+    }
+    if (threadIdx.x == 0 && atomicAdd(&p.counters[1], 1u) == gridDim.x - 1) {
+        atomicExch(&p.counters[0], 0u);
+        atomicExch(&p.counters[1], 0u);
+    }
+}
+
+// ---------------------------------------------------------- tuning table --
+struct Key {
+    int mode, bm, bn, bk;
+    bool operator<(const Key& o) const {
+        return std::tie(mode, bm, bn, bk) < std::tie(o.mode, o.bm, o.bn, o.bk);
+    }
+};
+std::mutex g_tune_mu;
+std::map<Key, int>& tune_table() {
+    static auto* t = new std::map<Key, int>();
+    return *t;
+}
+
+int lg(int64_t x) {
+    int b = 0;
+    while (x > 1) {
+        x >>= 1;
+        ++b;
+    }
+    return b;
+}
+
+Key key_of(int mode, const omb_gemm_desc& d) { return Key{mode, lg(d.M), lg(d.N), lg(d.K)}; }
+
+// Class without a tuning entry: split mode has one class; plain picks the
+// largest tile that the smaller output dimension fills.
+int default_class(int mode, const omb_gemm_desc& d) {
+    if (mode != OMB_MODE_PLAIN_F32) return CSPLIT;
+    const int64_t s = std::min(d.M, d.N);
+    if (s < 48) return C32;
+    if (s < 192) return C64;
+    return C128x64;
+}
+
+int choose_class(int mode, const omb_gemm_desc& d) {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto it = tune_table().find(key_of(mode, d));
+    return it != tune_table().end() ? it->second : default_class(mode, d);
+}
+
+}  // namespace
+}  // namespace omb
+
+// ------------------------------------------------------------------ plans --
+struct omb_group_plan {
+    int dev = 0;
+    int mode = 0;
+    bool big = false;
+    int grid = 0;
+    int nitems = 0;
+    omb::DGemm* d_gemms = nullptr;
+    omb::DGemv* d_gemvs = nullptr;
+    omb::Item* d_items = nullptr;
+    unsigned* d_counters = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::string text;
+};
+
+namespace omb {
+namespace {
+
+omb_status validate(const omb_gemm_desc* gemms, int32_t n_gemm, const omb_gemv_desc* gemvs,
+                    int32_t n_gemv) {
+    if (n_gemm < 0 || n_gemv < 0 || (n_gemm > 0 && !gemms) || (n_gemv > 0 && !gemvs)) {
+        set_error("group plan: bad descriptor array");
+        return OMB_EINVAL;
+    }
+    for (int i = 0; i < n_gemm; ++i) {
+        const omb_gemm_desc& d = gemms[i];
+        const bool empty = d.M == 0 || d.N == 0;
+        if (d.M < 0 || d.N < 0 || d.K < 0 || (!empty && (!d.C || d.ldc < d.N)) ||
+            (!empty && d.K > 0 && (!d.A || !d.B || d.lda < d.K || d.ldb < d.N))) {
+            set_error("group plan: invalid GEMM descriptor " + std::to_string(i));
+            return OMB_EINVAL;
+        }
+    }
+    for (int i = 0; i < n_gemv; ++i) {
+        const omb_gemv_desc& d = gemvs[i];
+        if (d.M < 0 || d.N < 0 || (d.M > 0 && !d.y) ||
+            (d.M > 0 && d.N > 0 && (!d.A || !d.x || d.lda < d.N))) {
+            set_error("group plan: invalid GEMV descriptor " + std::to_string(i));
+            return OMB_EINVAL;
+        }
+    }
+    return OMB_OK;
+}
+
+// Estimated seconds of one workgroup on one CU (for longest-first order).
+double item_cost(const Item& it, const std::vector<DGemm>& g, const std::vector<DGemv>& v) {
+    if (it.cls == CGEMV) return 64.0 * (double)v[it.inst].N * 4.0 / 31e9;
+    const double rate = it.cls == CSPLIT ? 2.0e12 : 0.31e12;
+    return 2.0 * kTileM[it.cls] * kTileN[it.cls] * (double)std::max<int64_t>(g[it.inst].K, 1) / rate;
+}
+
+template <typename T>
+omb_status upload(T** dst, const std::vector<T>& src) {
+    *dst = nullptr;
+    if (src.empty()) return OMB_OK;
+    OMB_HIP_TRY(hipMalloc(reinterpret_cast<void**>(dst), src.size() * sizeof(T)));
+    OMB_HIP_TRY(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    return OMB_OK;
+}
+
+void free_plan(omb_group_plan* p) {
+    if (!p) return;
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->d_gemms) (void)hipFree(p->d_gemms);
+    if (p->d_gemvs) (void)hipFree(p->d_gemvs);
+    if (p->d_items) (void)hipFree(p->d_items);
+    if (p->d_counters) (void)hipFree(p->d_counters);
+    delete p;
+}
+
+omb_status launch_plan(omb_group_plan* p, hipStream_t s) {
+    if (p->nitems == 0) return OMB_OK;
+    PArgs a{p->d_items, p->nitems, p->d_gemms, p->d_gemvs, p->d_counters,
+            p->mode == OMB_MODE_SPLIT_BF16_P3 ? 3 : 6};
+    if (p->big)
+        hipLaunchKernelGGL(grouped_kernel<true>, dim3(p->grid), dim3(NT), 0, s, a);
+    else
+        hipLaunchKernelGGL(grouped_kernel<false>, dim3(p->grid), dim3(NT), 0, s, a);
+    OMB_HIP_TRY(hipGetLastError());
+    return OMB_OK;
+}
+
+// Build a plan; force_cls >= 0 puts every GEMM in that class (autotuner).
+omb_status build_plan(const omb_gemm_desc* gemms, int32_t n_gemm, const omb_gemv_desc* gemvs,
+                      int32_t n_gemv, int32_t mode, int force_cls, bool capture,
+                      omb_group_plan** out) {
+    *out = nullptr;
+    if (mode != OMB_MODE_PLAIN_F32 && mode != OMB_MODE_SPLIT_BF16 &&
+        mode != OMB_MODE_SPLIT_BF16_P3) {
+        set_error("group plan: unknown mode " + std::to_string(mode));
+        return OMB_EINVAL;
+    }
+    OMB_TRY(validate(gemms, n_gemm, gemvs, n_gemv));
+    auto* p = new omb_group_plan();
+    p->mode = mode;
+    std::vector<DGemm> g(n_gemm);
+    std::vector<DGemv> v(n_gemv);
+    std::vector<Item> items;
+    int per_cls[NCLS] = {0, 0, 0, 0, 0};
+    std::map<std::string, int> chosen;
+    for (int i = 0; i < n_gemm; ++i) {
+        const omb_gemm_desc& d = gemms[i];
+        g[i] = DGemm{d.M, d.N, d.K, d.A, d.lda, d.B, d.ldb, d.C, d.ldc, d.alpha, d.beta};
+        if (d.M == 0 || d.N == 0) continue;
+        const int c = force_cls >= 0 ? force_cls : choose_class(mode, d);
+        const int64_t tmn = ceil_div(d.M, kTileM[c]), tnn = ceil_div(d.N, kTileN[c]);
+        if (tmn > 0x7fffffff || tnn > 0x7fffffff || (int64_t)items.size() + tmn * tnn > 0x7fffffff) {
+            free_plan(p);
+            set_error("group plan: too many tiles");
+            return OMB_EUNSUPPORTED;
+        }
+        for (int64_t a = 0; a < tmn; ++a)
+            for (int64_t b = 0; b < tnn; ++b) items.push_back(Item{c, i, (int)a, (int)b});
+        per_cls[c] += (int)(tmn * tnn);
+        if (c == C128x64) p->big = true;
+        const Key k = key_of(mode, d);
+        chosen["gemm:" + std::to_string(1 << k.bm) + "x" + std::to_string(1 << k.bn) + "x" +
+               std::to_string(1 << k.bk) + "=" + kClsName[c]] += 1;
+    }
+    for (int i = 0; i < n_gemv; ++i) {
+        const omb_gemv_desc& d = gemvs[i];
+        v[i] = DGemv{d.M, d.N, d.A, d.lda, d.x, d.y, d.alpha, d.beta,
+                     gemv_row_vec(d.A, d.lda, d.x) ? 1 : 0};
+        for (int64_t c = 0; c < ceil_div(d.M, 64); ++c) items.push_back(Item{CGEMV, i, (int)c, 0});
+        per_cls[CGEMV] += (int)ceil_div(d.M, 64);
+    }
+    std::vector<double> cost(items.size());
+    for (size_t i = 0; i < items.size(); ++i) cost[i] = item_cost(items[i], g, v);
+    std::vector<size_t> order(items.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+    std::vector<Item> sorted(items.size());
+    for (size_t i = 0; i < order.size(); ++i) sorted[i] = items[order[i]];
+    p->nitems = (int)sorted.size();
+
+    omb_status st = OMB_OK;
+    if ((st = upload(&p->d_gemms, g)) != OMB_OK || (st = upload(&p->d_gemvs, v)) != OMB_OK ||
+        (st = upload(&p->d_items, sorted)) != OMB_OK) {
+        free_plan(p);
+        return st;
+    }
+    if (hipMalloc(&p->d_counters, 2 * sizeof(unsigned)) != hipSuccess ||
+        hipMemset(p->d_counters, 0, 2 * sizeof(unsigned)) != hipSuccess) {
+        (void)hipGetLastError();
+        free_plan(p);
+        set_error("group plan: counter allocation failed");
+        return OMB_ENOMEM;
+    }
+    int dev = 0, cus = 0, per_cu = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        free_plan(p);
+        set_error("group plan: device query failed");
+        return OMB_EDEVICE;
+    }
+    cus = prop.multiProcessorCount;
+    hipError_t e = p->big ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, grouped_kernel<true>, NT, 0)
+                          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, grouped_kernel<false>, NT, 0);
+    if (e != hipSuccess || per_cu < 1) per_cu = 1;
+    p->dev = dev;
+    p->grid = std::max(1, std::min(p->nitems, cus * per_cu));
+
+    if (capture && p->nitems > 0) {
+        hipStream_t cs = nullptr;
+        if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
+            free_plan(p);
+            set_error("group plan: stream creation failed");
+            return OMB_EDEVICE;
+        }
+        bool ok = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        ok = ok && launch_plan(p, cs) == OMB_OK;
+        ok = hipStreamEndCapture(cs, &p->graph) == hipSuccess && ok;
+        ok = ok && hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0) == hipSuccess;
+        (void)hipStreamDestroy(cs);
+        if (!ok) {
+            (void)hipGetLastError();
+            free_plan(p);
+            set_error("group plan: graph capture failed");
+            return OMB_EDEVICE;
+        }
+    }
+    std::ostringstream os;
+    os << "items=" << p->nitems << " grid=" << p->grid << " kernel=" << (p->big ? "big" : "lite")
+       << " mode=" << mode;
+    for (int c = 0; c < NCLS; ++c)
+        if (per_cls[c]) os << " " << kClsName[c] << "=" << per_cls[c];
+    for (auto& kv : chosen) os << " " << kv.first << "(" << kv.second << ")";
+    p->text = os.str();
+    *out = p;
+    return OMB_OK;
+}
+
+}  // namespace
+}  // namespace omb
+
+using namespace omb;
+
+extern "C" omb_status omb_group_plan_create(const omb_gemm_desc* gemms, int32_t n_gemm,
+                                            const omb_gemv_desc* gemvs, int32_t n_gemv,
+                                            int32_t mode, omb_stream_t stream,
+                                            omb_group_plan** plan) {
+    clear_error();
+    (void)stream;  // the graph is captured on a private stream and replayed on any
+    if (!plan) {
+        set_error("omb_group_plan_create: null plan pointer");
+        return OMB_EINVAL;
+    }
+    return build_plan(gemms, n_gemm, gemvs, n_gemv, mode, -1, true, plan);
+}
+
+extern "C" omb_status omb_group_plan_launch(omb_group_plan* plan, omb_stream_t stream) {
+    clear_error();
+    if (!plan) {
+        set_error("omb_group_plan_launch: null plan");
+        return OMB_EINVAL;
+    }
+    if (plan->nitems == 0) return OMB_OK;
+    OMB_HIP_TRY(hipGraphLaunch(plan->exec, (hipStream_t)stream));
+    return OMB_OK;
+}
+
+extern "C" omb_status omb_group_plan_launch_direct(omb_group_plan* plan, omb_stream_t stream) {
+    clear_error();
+    if (!plan) {
+        set_error("omb_group_plan_launch_direct: null plan");
+        return OMB_EINVAL;
+    }
+    return launch_plan(plan, (hipStream_t)stream);
+}
+
+extern "C" omb_status omb_group_plan_describe(const omb_group_plan* plan, char* buf, size_t len) {
+    clear_error();
+    if (!plan || !buf || len == 0) {
+        set_error("omb_group_plan_describe: invalid argument");
+        return OMB_EINVAL;
+    }
+    std::snprintf(buf, len, "%s", plan->text.c_str());
+    return OMB_OK;
+}
+
+extern "C" omb_status omb_group_plan_destroy(omb_group_plan* plan) {
+    clear_error();
+    free_plan(plan);
+    return OMB_OK;
+}
+
+// ---------------------------------------------------------------- tuning --
+namespace omb {
+namespace {
+
+__global__ void tune_fill(float* p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        p[i] = rng_uniform(0x5eedULL, (uint64_t)i);
+}
+
+// Time `reps` direct launches of a plan that puts `batch` copies of the
+// shape in class `cls`; returns milliseconds per launch.
+omb_status time_class(int mode, int64_t M, int64_t N, int64_t K, int cls, int batch, float* A,
+                      float* B, float* C, hipStream_t s, float* ms) {
+    std::vector<omb_gemm_desc> d(batch);
+    for (int i = 0; i < batch; ++i)
+        d[i] = omb_gemm_desc{M, N, K, A, K, B, N, C + (int64_t)i * M * N, N, 1.f, 0.f};
+    omb_group_plan* p = nullptr;
+    OMB_TRY(build_plan(d.data(), batch, nullptr, 0, mode, cls, false, &p));
+    hipEvent_t e0, e1;
+    omb_status st = OMB_OK;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        free_plan(p);
+        set_error("autotune: event creation failed");
+        return OMB_EDEVICE;
+    }
+    for (int w = 0; w < 2 && st == OMB_OK; ++w) st = launch_plan(p, s);
+    const int reps = 5;
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < reps && st == OMB_OK; ++r) st = launch_plan(p, s);
+    (void)hipEventRecord(e1, s);
+    if (st == OMB_OK && hipEventSynchronize(e1) == hipSuccess) {
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, e0, e1);
+        *ms = t / reps;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    free_plan(p);
+    return st;
+}
+
+}  // namespace
+}  // namespace omb
+
+extern "C" omb_status omb_autotune_group(const omb_gemm_desc* gemms, int32_t n_gemm, int32_t mode,
+                                         const char* table_path, omb_stream_t stream) {
+    clear_error();
+    OMB_TRY(validate(gemms, n_gemm, nullptr, 0));
+    if (mode != OMB_MODE_PLAIN_F32 && mode != OMB_MODE_SPLIT_BF16 &&
+        mode != OMB_MODE_SPLIT_BF16_P3) {
+        set_error("omb_autotune_group: unknown mode");
+        return OMB_EINVAL;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    // bucket -> shapes present (the median M*N*K shape represents the bucket)
+    std::map<Key, std::vector<omb_gemm_desc>> buckets;
+    for (int i = 0; i < n_gemm; ++i)
+        if (gemms[i].M > 0 && gemms[i].N > 0) buckets[key_of(mode, gemms[i])].push_back(gemms[i]);
+    const int batch = 64;
+    const std::vector<int> cands = mode == OMB_MODE_PLAIN_F32
+                                       ? std::vector<int>{C32, C64, C128x64}
+                                       : std::vector<int>{CSPLIT};
+    int64_t maxM = 1, maxN = 1, maxK = 1;
+    for (auto& kv : buckets)
+        for (auto& d : kv.second) {
+            maxM = std::max(maxM, d.M);
+            maxN = std::max(maxN, d.N);
+            maxK = std::max(maxK, d.K);
+        }
+    float *A = nullptr, *B = nullptr, *C = nullptr;
+    if (cands.size() > 1 && !buckets.empty()) {
+        if (hipMalloc(&A, maxM * maxK * sizeof(float)) != hipSuccess ||
+            hipMalloc(&B, maxK * maxN * sizeof(float)) != hipSuccess ||
+            hipMalloc(&C, (size_t)batch * maxM * maxN * sizeof(float)) != hipSuccess) {
+            (void)hipGetLastError();
+            if (A) (void)hipFree(A);
+            if (B) (void)hipFree(B);
+            set_error("omb_autotune_group: scratch allocation failed");
+            return OMB_ENOMEM;
+        }
+        hipLaunchKernelGGL(tune_fill, dim3(1024), dim3(256), 0, s, A, maxM * maxK);
+        hipLaunchKernelGGL(tune_fill, dim3(1024), dim3(256), 0, s, B, maxK * maxN);
+    }
+    omb_status st = OMB_OK;
+    std::map<Key, int> result;
+    for (auto& kv : buckets) {
+        if (cands.size() == 1) {
+            result[kv.first] = cands[0];
+            continue;
+        }
+        auto shapes = kv.second;
+        std::sort(shapes.begin(), shapes.end(), [](const omb_gemm_desc& a, const omb_gemm_desc& b) {
+            return a.M * a.N * a.K < b.M * b.N * b.K;
+        });
+        const omb_gemm_desc rep = shapes[shapes.size() / 2];
+        int best = cands[0];
+        float best_ms = 1e30f;
+        for (int c : cands) {
+            float ms = 1e30f;
+            st = time_class(mode, rep.M, rep.N, rep.K, c, batch, A, B, C, s, &ms);
+            if (st != OMB_OK) break;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = c;
+            }
+        }
+        if (st != OMB_OK) break;
+        result[kv.first] = best;
+    }
+    if (A) (void)hipFree(A);
+    if (B) (void)hipFree(B);
+    if (C) (void)hipFree(C);
+    if (st != OMB_OK) return st;
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        for (auto& kv : result) tune_table()[kv.first] = kv.second;
+    }
+    if (table_path) {
+        std::ofstream f(table_path);
+        if (!f) {
+            set_error(std::string("omb_autotune_group: cannot write ") + table_path);
+            return OMB_EINVAL;
+        }
+        f << "# mode log2M log2N log2K class\n";
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        for (auto& kv : tune_table())
+            f << kv.first.mode << " " << kv.first.bm << " " << kv.first.bn << " " << kv.first.bk
+              << " " << kClsName[kv.second] << "\n";
+    }
+    return OMB_OK;
+}
+
+extern "C" omb_status omb_autotune_load(const char* table_path) {
+    clear_error();
+    if (!table_path) {
+        set_error("omb_autotune_load: null path");
+        return OMB_EINVAL;
+    }
+    std::ifstream f(table_path);
+    if (!f) {
+        set_error(std::string("omb_autotune_load: cannot read ") + table_path);
+        return OMB_EINVAL;
+    }
+    std::map<Key, int> loaded;
+    std::string line;
+    while (std::getline(f, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::istringstream is(line);
+        Key k{};
+        std::string name;
+        if (!(is >> k.mode >> k.bm >> k.bn >> k.bk >> name)) {
+            set_error("omb_autotune_load: malformed line: " + line);
+            return OMB_EINVAL;
+        }
+        int cls = -1;
+        for (int c = 0; c < NCLS; ++c)
+            if (name == kClsName[c]) cls = c;
+        if (cls < 0 || cls == CGEMV) {
+            set_error("omb_autotune_load: unknown class " + name);
+            return OMB_EINVAL;
+        }
+        loaded[k] = cls;
+    }
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    for (auto& kv : loaded) tune_table()[kv.first] = kv.second;
+    return OMB_OK;
+}

@@ -194,7 +194,7 @@ __global__ __launch_bounds__(NT, 2) void sgemm_bf16x3_kernel(MArgs p) {
         for (int j = 0; j < 2; ++j) inner[i][j] = outer[i][j] = f32x16{};
     const int frow = lane & 31, fh = lane >> 5;
     auto compute = [&](int buf, int64_t kt) {
-        mfma_slice<NPROD>(lds[buf], wm, wn, frow, fh, inner);
+        mfma_slice<NPROD>(lds[buf], wm * 64, wn * 64, frow, fh, inner);
     };
     auto fold_at = [&](int64_t kt) {
         if ((kt + 1) % kFoldSlices == 0 || kt + 1 == p.nkt) fold(inner, outer);

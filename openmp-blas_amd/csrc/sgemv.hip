@@ -13,14 +13,12 @@
 // never on scheduling.
 #include <algorithm>
 
-#include "common.hpp"
+#include "gemv_row.hpp"
 
 namespace omb {
 namespace {
 
 constexpr int NT = 256;
-constexpr int ROWS = 4;    // rows per wavefront (row-major kernel)
-constexpr int UNROLL = 4;  // float4 loads in flight per row per lane
 
 template <bool VEC>
 __global__ __launch_bounds__(NT) void sgemv_row_kernel(int64_t M, int64_t N, float alpha,
@@ -29,73 +27,10 @@ __global__ __launch_bounds__(NT) void sgemv_row_kernel(int64_t M, int64_t N, flo
                                                        float* __restrict__ y) {
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
-    const int64_t r0 = wave * ROWS;
+    const int64_t r0 = wave * kGemvRows;
     if (r0 >= M) return;
-    const int nr = (int)std::min<int64_t>(ROWS, M - r0);
-    float acc[ROWS];
-#pragma unroll
-    for (int r = 0; r < ROWS; ++r) acc[r] = 0.f;
-
-    int64_t done = 0;
-    if (VEC) {
-        const int64_t n4 = N >> 2;
-        const float4* x4 = reinterpret_cast<const float4*>(x);
-        const float4* a4[ROWS];
-#pragma unroll
-        for (int r = 0; r < ROWS; ++r)
-            a4[r] = reinterpret_cast<const float4*>(A + (r0 + std::min(r, nr - 1)) * lda);
-        int64_t j = lane;
-        for (; j + 64 * (UNROLL - 1) < n4; j += 64 * UNROLL) {
-            float4 xv[UNROLL];
-            float4 av[ROWS][UNROLL];
-#pragma unroll
-            for (int u = 0; u < UNROLL; ++u) xv[u] = x4[j + 64 * u];
-#pragma unroll
-            for (int r = 0; r < ROWS; ++r)
-#pragma unroll
-                for (int u = 0; u < UNROLL; ++u)
-                    av[r][u] = nt_load4(&a4[r][j + 64 * u]);
-#pragma unroll
-            for (int u = 0; u < UNROLL; ++u)
-#pragma unroll
-                for (int r = 0; r < ROWS; ++r) {
-                    acc[r] = fmaf(av[r][u].x, xv[u].x, acc[r]);
-                    acc[r] = fmaf(av[r][u].y, xv[u].y, acc[r]);
-                    acc[r] = fmaf(av[r][u].z, xv[u].z, acc[r]);
-                    acc[r] = fmaf(av[r][u].w, xv[u].w, acc[r]);
-                }
-        }
-        for (; j < n4; j += 64) {
-            float4 xv = x4[j];
-#pragma unroll
-            for (int r = 0; r < ROWS; ++r) {
-                float4 av = nt_load4(&a4[r][j]);
-                acc[r] = fmaf(av.x, xv.x, acc[r]);
-                acc[r] = fmaf(av.y, xv.y, acc[r]);
-                acc[r] = fmaf(av.z, xv.z, acc[r]);
-                acc[r] = fmaf(av.w, xv.w, acc[r]);
-            }
-        }
-        done = n4 << 2;
-    }
-    for (int64_t j = done + lane; j < N; j += 64) {
-        float xv = x[j];
-#pragma unroll
-        for (int r = 0; r < ROWS; ++r) {
-            if (r < nr) acc[r] = fmaf(A[(r0 + r) * lda + j], xv, acc[r]);
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < ROWS; ++r) acc[r] = wave_sum(acc[r]);
-    if (lane < nr) {
-        float s = acc[0];
-#pragma unroll
-        for (int r = 1; r < ROWS; ++r)
-            if (lane == r) s = acc[r];
-        float out = alpha * s;
-        if (beta != 0.f) out += beta * y[r0 + lane];
-        y[r0 + lane] = out;
-    }
+    const int nr = (int)std::min<int64_t>(kGemvRows, M - r0);
+    gemv_rows_wave<VEC>(r0, nr, N, alpha, A, lda, x, beta, y, lane);
 }
 
 constexpr int CRPT = 16;                 // rows per lane (4 float4)
@@ -236,15 +171,13 @@ __global__ __launch_bounds__(NT) void sgemv_col_combine_kernel(int64_t M, float 
 omb_status launch_sgemv_row(int64_t M, int64_t N, float alpha, const float* A, int64_t lda,
                             const float* x, float beta, float* y, hipStream_t s) {
     if (M <= 0) return OMB_OK;
-    const int64_t waves = ceil_div(M, ROWS);
+    const int64_t waves = ceil_div(M, kGemvRows);
     const int64_t blocks = ceil_div(waves, NT / 64);
     if (blocks > 0x7fffffffLL) {
         set_error("sgemv: too many rows");
         return OMB_EUNSUPPORTED;
     }
-    const bool vec = ((reinterpret_cast<uintptr_t>(A) & 15) == 0) &&
-                     ((reinterpret_cast<uintptr_t>(x) & 15) == 0) && (lda % 4 == 0);
-    if (vec)
+    if (gemv_row_vec(A, lda, x))
         hipLaunchKernelGGL(sgemv_row_kernel<true>, dim3((unsigned)blocks), dim3(NT), 0, s, M, N,
                            alpha, A, lda, x, beta, y);
     else

@@ -60,29 +60,34 @@ __device__ inline void split8(float4 lo, float4 hi, uint4& p0, uint4& p1, uint4&
 namespace omb {
 namespace split {
 
-// MFMAs of one k slice for one wavefront (64x64 of C as 2x2 blocks of 32x32):
-// fragments from the swizzled LDS slice, smallest products first, all into
-// the inner accumulators.
-template <int NPROD>
-__device__ __forceinline__ void mfma_slice(const uint4* slice, int wm, int wn, int frow, int fh,
-                                           f32x16 (&inner)[2][2]) {
+// MFMAs of one k slice for one wavefront owning BI x BJ blocks of 32x32 of C
+// (rows wrow0.., columns wcol0.. of the slice): fragments from the swizzled
+// LDS slice (PA rows per A plane, PB per B plane), smallest products first,
+// all into the inner accumulators.
+template <int NPROD, int BI = 2, int BJ = 2, int PA = TM, int PB = TN>
+__device__ __forceinline__ void mfma_slice(const uint4* slice, int wrow0, int wcol0, int frow,
+                                           int fh, f32x16 (&inner)[BI][BJ]) {
     constexpr int NPLANES = NPROD == 6 ? 3 : 2;
     const uint4* la = slice;
-    const uint4* lb = slice + SLICE_A / 8;
-    bf16x8 a[NPLANES][2], b[NPLANES][2];
+    const uint4* lb = slice + 3 * 2 * PA;
+    bf16x8 a[NPLANES][BI], b[NPLANES][BJ];
 #pragma unroll
-    for (int pl = 0; pl < NPLANES; ++pl)
+    for (int pl = 0; pl < NPLANES; ++pl) {
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            uint4 va = la[pl * 2 * TM + lds_chunk(wm * 64 + i * 32 + frow, fh)];
-            uint4 vb = lb[pl * 2 * TN + lds_chunk(wn * 64 + i * 32 + frow, fh)];
+        for (int i = 0; i < BI; ++i) {
+            uint4 va = la[pl * 2 * PA + lds_chunk(wrow0 + i * 32 + frow, fh)];
             a[pl][i] = *reinterpret_cast<const bf16x8*>(&va);
-            b[pl][i] = *reinterpret_cast<const bf16x8*>(&vb);
         }
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < BJ; ++j) {
+            uint4 vb = lb[pl * 2 * PB + lds_chunk(wcol0 + j * 32 + frow, fh)];
+            b[pl][j] = *reinterpret_cast<const bf16x8*>(&vb);
+        }
+    }
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+    for (int i = 0; i < BI; ++i)
+#pragma unroll
+        for (int j = 0; j < BJ; ++j) {
             if (NPROD == 6) {
                 inner[i][j] = mfma(a[0][i], b[2][j], inner[i][j]);
                 inner[i][j] = mfma(a[1][i], b[1][j], inner[i][j]);
@@ -96,26 +101,28 @@ __device__ __forceinline__ void mfma_slice(const uint4* slice, int wm, int wn, i
 
 // outer += inner; inner = 0 (FP32 adds: the running sum over K is rounded
 // by the vector ALUs, not inside the MFMA accumulation).
-__device__ __forceinline__ void fold(f32x16 (&inner)[2][2], f32x16 (&outer)[2][2]) {
+template <int BI, int BJ>
+__device__ __forceinline__ void fold(f32x16 (&inner)[BI][BJ], f32x16 (&outer)[BI][BJ]) {
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < BI; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < BJ; ++j) {
             outer[i][j] += inner[i][j];
             inner[i][j] = f32x16{};
         }
 }
 
-// C = alpha * acc + beta * C for the wavefront's 64x64 block (row-major C).
+// C = alpha * acc + beta * C for the wavefront's BI x BJ blocks (row-major C).
 // 32x32 accumulator layout: lane l holds column l % 32, value r holds row
 // 8*(r/4) + 4*(l/32) + r%4.
-__device__ __forceinline__ void store_tile(const f32x16 (&acc)[2][2], int64_t m_base,
+template <int BI, int BJ>
+__device__ __forceinline__ void store_tile(const f32x16 (&acc)[BI][BJ], int64_t m_base,
                                            int64_t n_base, int lane, float* C, int64_t ldc,
                                            int64_t M, int64_t N, float alpha, float beta) {
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < BI; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < BJ; ++j) {
             const int64_t n = n_base + j * 32 + (lane & 31);
             if (n >= N) continue;
 #pragma unroll

@@ -97,3 +97,20 @@ def test_oracle_is_not_linked_into_the_product():
     syms = subprocess.run(["nm", "-D", helpers.graft.load_package().LIB_PATH],
                           capture_output=True, text=True).stdout
     assert "oracle_" not in syms
+
+
+def test_group_plan_rejects_bad_descriptors(ompblas):
+    """Descriptor validation runs before any device call."""
+    import ctypes
+    lib = ompblas.lib()
+    h = ctypes.c_void_p()
+    bad = (ompblas.GemmDesc * 1)(ompblas.GemmDesc(4, 4, 4, 16, 2, 16, 4, 16, 4, 1.0, 0.0))  # lda < K
+    assert lib.omb_group_plan_create(bad, 1, None, 0, 0, None, ctypes.byref(h)) == -1
+    neg = (ompblas.GemmDesc * 1)(ompblas.GemmDesc(-1, 4, 4, 16, 4, 16, 4, 16, 4, 1.0, 0.0))
+    assert lib.omb_group_plan_create(neg, 1, None, 0, 0, None, ctypes.byref(h)) == -1
+    badv = (ompblas.GemvDesc * 1)(ompblas.GemvDesc(4, 8, 16, 4, 16, 16, 1.0, 0.0))       # lda < N
+    assert lib.omb_group_plan_create(None, 0, badv, 1, 0, None, ctypes.byref(h)) == -1
+    assert lib.omb_group_plan_create(None, 0, None, 0, 9, None, ctypes.byref(h)) == -1   # mode
+    assert lib.omb_group_plan_create(None, -1, None, 0, 0, None, ctypes.byref(h)) == -1
+    assert "group plan" in ompblas.last_error()
+    assert lib.omb_autotune_load(b"/nonexistent/table.txt") == -1

@@ -1,0 +1,155 @@
+"""Grouped persistent launch (one kernel, hipGraph) against per-instance
+omb_sgemm / omb_sgemv: bitwise equal in every mode, on every graph replay."""
+import math
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import helpers
+
+pytestmark = pytest.mark.gpu
+
+
+def log_uniform_ints(rng, n, lo, hi):
+    """floor(exp(U[ln lo, ln(hi+1)))) -- the configs' log-uniform sizes."""
+    return np.floor(np.exp(rng.uniform(math.log(lo), math.log(hi + 1), n))).astype(np.int64)
+
+
+class Batch:
+    """Device operands for a batch of row-major GEMMs and GEMVs (one buffer
+    per operand kind, 16-byte aligned slices)."""
+
+    def __init__(self, dev, ompblas, gemm_shapes, gemv_shapes, seed, beta=0.0):
+        def alloc(sizes):
+            offs, tot = [], 0
+            for s in sizes:
+                offs.append(tot)
+                tot += (int(s) + 3) // 4 * 4
+            buf = torch.empty(max(tot, 4), device=dev)
+            return buf, offs
+
+        self.ga, self.oa = alloc([m * k for m, n, k in gemm_shapes])
+        self.gb, self.ob = alloc([k * n for m, n, k in gemm_shapes])
+        self.gc, self.oc = alloc([m * n for m, n, k in gemm_shapes])
+        self.va, self.ova = alloc([m * n for m, n in gemv_shapes])
+        self.vx, self.ovx = alloc([n for m, n in gemv_shapes])
+        self.vy, self.ovy = alloc([m for m, n in gemv_shapes])
+        for i, t in enumerate((self.ga, self.gb, self.gc, self.va, self.vx, self.vy)):
+            ompblas.fill_uniform(t, seed, i)
+        self.c0 = self.gc.clone()
+        self.y0 = self.vy.clone()
+        self.gemm_shapes, self.gemv_shapes, self.beta = gemm_shapes, gemv_shapes, beta
+
+    def gemm_descs(self, alpha=1.0):
+        p = lambda t, o: t.data_ptr() + 4 * o
+        return [(m, n, k, p(self.ga, self.oa[i]), k, p(self.gb, self.ob[i]), n,
+                 p(self.gc, self.oc[i]), n, alpha, self.beta)
+                for i, (m, n, k) in enumerate(self.gemm_shapes)]
+
+    def gemv_descs(self, alpha=1.0):
+        p = lambda t, o: t.data_ptr() + 4 * o
+        return [(m, n, p(self.va, self.ova[i]), n, p(self.vx, self.ovx[i]), p(self.vy, self.ovy[i]),
+                 alpha, self.beta) for i, (m, n) in enumerate(self.gemv_shapes)]
+
+    def reset(self):
+        self.gc.copy_(self.c0)
+        self.vy.copy_(self.y0)
+
+    def eager(self, ompblas, mode):
+        """Per-instance calls (the reference semantics, one call each)."""
+        self.reset()
+        for d in self.gemm_descs():
+            m, n, k, a, lda, b, ldb, c, ldc, al, be = d
+            ompblas.check(ompblas.sgemm(m, n, k, al, a, lda, 1, b, ldb, 1, be, c, ldc, 1, mode))
+        for d in self.gemv_descs():
+            m, n, a, lda, x, y, al, be = d
+            ompblas.check(ompblas.sgemv(m, n, al, a, lda, 1, x, be, y))
+        torch.cuda.synchronize()
+        return self.gc.clone(), self.vy.clone()
+
+
+def shapes(seed, n_gemm, n_gemv, lo=16, hi=512, vlo=64, vhi=4096):
+    rng = np.random.default_rng(seed)
+    g = list(zip(*(log_uniform_ints(rng, n_gemm, lo, hi) for _ in range(3))))
+    v = list(zip(*(log_uniform_ints(rng, n_gemv, vlo, vhi) for _ in range(2))))
+    return [tuple(map(int, s)) for s in g], [tuple(map(int, s)) for s in v]
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_grouped_bitwise_equals_individual_calls(dev, ompblas, mode):
+    gs, vs = shapes(3, 300, 120)
+    b = Batch(dev, ompblas, gs, vs, seed=3)
+    want_c, want_y = b.eager(ompblas, mode)
+    plan = ompblas.GroupPlan(b.gemm_descs(), b.gemv_descs(), mode)
+    print(plan.describe())
+    for replay in range(3):          # eager work and a sync between replays
+        b.reset()
+        torch.cuda.synchronize()
+        plan.launch()
+        torch.cuda.synchronize()
+        assert torch.equal(b.gc, want_c), (mode, replay)
+        assert torch.equal(b.vy, want_y), (mode, replay)
+    b.reset()
+    plan.launch_direct()
+    torch.cuda.synchronize()
+    assert torch.equal(b.gc, want_c) and torch.equal(b.vy, want_y)
+    plan.close()
+
+
+def test_grouped_beta_alpha_and_empty_instances(dev, ompblas):
+    gs = [(33, 17, 0), (0, 5, 7), (5, 0, 7), (40, 70, 9), (1, 1, 1), (512, 3, 100)]
+    vs = [(70, 0), (0, 10), (130, 257), (1, 1)]
+    b = Batch(dev, ompblas, gs, vs, seed=5, beta=0.5)
+    want_c, want_y = b.eager(ompblas, 0)
+    b.reset()
+    plan = ompblas.GroupPlan(b.gemm_descs(), b.gemv_descs(), 0)
+    plan.launch()
+    torch.cuda.synchronize()
+    assert torch.equal(b.gc, want_c) and torch.equal(b.vy, want_y)
+    plan.close()
+
+
+def test_autotuner_table_roundtrip(dev, ompblas, tmp_path):
+    gs, vs = shapes(7, 120, 0)
+    b = Batch(dev, ompblas, gs, vs, seed=7)
+    path = str(tmp_path / "tiles.txt")
+    ompblas.autotune_group(b.gemm_descs(), 0, path)
+    lines = [l for l in open(path) if not l.startswith("#")]
+    assert len(lines) >= 1 and all(len(l.split()) == 5 for l in lines)
+    ompblas.autotune_load(path)
+    want_c, want_y = b.eager(ompblas, 0)
+    b.reset()
+    plan = ompblas.GroupPlan(b.gemm_descs(), [], 0)
+    desc = plan.describe()
+    assert "gemm:" in desc
+    plan.launch()
+    torch.cuda.synchronize()
+    assert torch.equal(b.gc, want_c)
+    plan.close()
+
+
+def test_grouped_integer_exact(dev, ompblas):
+    """Integer operands (rand()%10): every GEMM / GEMV exact in FP32."""
+    gs, vs = shapes(11, 64, 32, hi=200, vhi=1000)
+    b = Batch(dev, ompblas, gs, vs, seed=11)
+    for t in (b.ga, b.gb, b.va, b.vx):
+        t.copy_(torch.from_numpy(helpers.rand100((t.numel(),)) % 10).to(dev))
+    b.c0.zero_()
+    b.y0.zero_()
+    b.reset()
+    plan = ompblas.GroupPlan(b.gemm_descs(), b.gemv_descs(), 0)
+    plan.launch()
+    torch.cuda.synchronize()
+    for i, (m, n, k) in enumerate(gs):
+        a = b.ga[b.oa[i]:b.oa[i] + m * k].view(m, k).double()
+        bb = b.gb[b.ob[i]:b.ob[i] + k * n].view(k, n).double()
+        c = b.gc[b.oc[i]:b.oc[i] + m * n].view(m, n).double()
+        assert torch.equal(c, a @ bb), i
+    for i, (m, n) in enumerate(vs):
+        a = b.va[b.ova[i]:b.ova[i] + m * n].view(m, n).double()
+        x = b.vx[b.ovx[i]:b.ovx[i] + n].double()
+        y = b.vy[b.ovy[i]:b.ovy[i] + m].double()
+        assert torch.equal(y, a @ x), i
+    plan.close()
