@@ -65,6 +65,12 @@ def main():
     ap.add_argument("--no-errors", action="store_true", help="skip the FP64 error legs")
     ap.add_argument("--cpu-budget", type=float, default=25.0,
                     help="seconds of host time for the CPU baseline sample")
+    ap.add_argument("--serve-rate", type=float, default=float(os.environ.get("OMB_SERVE_RATE", 32000)),
+                    help="serving leg: Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-count", type=int, default=int(os.environ.get("OMB_SERVE_COUNT", 3000)),
+                    help="serving leg: requests in the trace")
+    ap.add_argument("--serve-slots", type=int, default=int(os.environ.get("OMB_SERVE_SLOTS", 8)),
+                    help="serving leg: requests in flight at most (one stream each)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -202,6 +208,13 @@ def main():
     del dx, dy
     sync()
 
+    # ------------------------------------------------ grouped and serving --
+    grouped = grouped_leg(pkg, lib, torch, dev, stream, timer, args.steps)
+    sync()
+    serving = serving_leg(pkg, torch, dev, stream, args.serve_rate, args.serve_count,
+                          args.serve_slots)
+    sync()
+
     # ------------------------------------------------------- CPU baseline --
     cpu = None
     if not args.no_cpu:
@@ -236,6 +249,8 @@ def main():
         "errors": errors,
         "gemv": gemv,
         "dot": dot,
+        "grouped": grouped,
+        "serving": serving,
         "kernels": kernels,
         "device": torch.cuda.get_device_name(0),
     }
@@ -292,6 +307,162 @@ def error_leg(pkg, lib, torch, dev, sp, A, B, C):
                                              d[name]["frob_rel"] / p["frob_rel"]]
         out[dist] = d
     del C64
+    return out
+
+
+def _pool(torch, dev, stream, sizes):
+    """One device buffer holding consecutive 16-byte aligned slices."""
+    sizes = np.asarray(sizes, dtype=np.int64)
+    padded = (sizes + 3) // 4 * 4
+    offs = np.concatenate([[0], np.cumsum(padded)[:-1]]).astype(np.int64)
+    with torch.cuda.stream(stream):
+        buf = torch.empty(max(int(padded.sum()), 4), device=dev)
+    return buf, offs
+
+
+def grouped_leg(pkg, lib, torch, dev, stream, timer, steps):
+    """4096 row-major GEMMs (M, N, K log-uniform on [16, 512]) and 4096 GEMVs
+    (rows, cols log-uniform on [64, 4096]) from seed 3, operands U[-1,1),
+    C = A*B and y = A*x, in one persistent grouped launch replayed from a
+    hipGraph; tiles from the autotuner run on the same batch.  Compared with
+    t_roof = sum over instances of max(flop / peak, bytes / 8 TB/s) and with
+    the same work as one omb_sgemm / omb_sgemv call per instance."""
+    log("grouped: 4096 GEMMs + 4096 GEMVs (seed 3)")
+    sp = stream.cuda_stream
+    rng = np.random.default_rng(3)
+    gs = np.stack([pkg.log_uniform_ints(rng, 16, 512, 4096) for _ in range(3)], 1)
+    vs = np.stack([pkg.log_uniform_ints(rng, 64, 4096, 4096) for _ in range(2)], 1)
+    M, N, K = gs[:, 0], gs[:, 1], gs[:, 2]
+    VM, VN = vs[:, 0], vs[:, 1]
+    pools = [_pool(torch, dev, stream, x) for x in (M * K, K * N, M * N, VM * VN, VN, VM)]
+    for i, (buf, _) in enumerate(pools):
+        pkg.fill_uniform(buf, 3, i, stream=sp)
+    ptr = lambda j, i: pools[j][0].data_ptr() + 4 * int(pools[j][1][i])
+    gemms = [(int(M[i]), int(N[i]), int(K[i]), ptr(0, i), int(K[i]), ptr(1, i), int(N[i]),
+              ptr(2, i), int(N[i]), 1.0, 0.0) for i in range(len(gs))]
+    gemvs = [(int(VM[i]), int(VN[i]), ptr(3, i), int(VN[i]), ptr(4, i), ptr(5, i), 1.0, 0.0)
+             for i in range(len(vs))]
+    g_flop = float((2.0 * M * N * K).sum())
+    g_bytes = 4.0 * (M * K + K * N + M * N)
+    v_bytes = 4.0 * (VM * VN + VM + VN)
+    v_flop = float((2.0 * VM * VN).sum())
+
+    def roof(fp32_rate):
+        tg = np.maximum(2.0 * M * N * K / fp32_rate, g_bytes / PEAK_HBM).sum()
+        tv = np.maximum(2.0 * VM * VN / fp32_rate, v_bytes / PEAK_HBM).sum()
+        return float(tg), float(tv)
+
+    t0 = time.perf_counter()
+    pkg.autotune_group(gemms, pkg.PLAIN_F32, None, stream=sp)
+    t_tune = time.perf_counter() - t0
+
+    def time_plan(g, v, mode):
+        plan = pkg.GroupPlan(g, v, mode, stream=sp)
+        for _ in range(2):
+            plan.launch(sp)
+        t = timer.run(lambda: plan.launch(sp), max(5, steps))
+        desc = plan.describe()
+        plan.close()
+        return t, desc
+
+    out = {"instances": {"gemm": len(gemms), "gemv": len(gemvs)},
+           "gemm_flop": g_flop, "gemv_flop": v_flop,
+           "gemm_bytes": float(g_bytes.sum()), "gemv_bytes": float(v_bytes.sum()),
+           "autotune_s": round(t_tune, 3)}
+    for mode, name, rate in ((pkg.PLAIN_F32, "plain_f32", PEAK_FP32),
+                             (pkg.SPLIT_BF16, "split_bf16_p6", PEAK_MFMA_BF16 / 6)):
+        t_all, desc = time_plan(gemms, gemvs, mode)
+        t_g, _ = time_plan(gemms, [], mode)
+        t_v, _ = time_plan([], gemvs, mode)
+        tg, tv = roof(rate)
+        classes = {}
+        for tok in desc.split():
+            if "=" in tok and tok.startswith("gemm:"):
+                cls = tok.split("=", 1)[1].split("(")[0]
+                cnt = int(tok.rsplit("(", 1)[1].rstrip(")")) if "(" in tok else 1
+                classes[cls] = classes.get(cls, 0) + cnt
+        out[name] = {"ms": t_all * 1e3, "t_roof_ms": (tg + tv) * 1e3,
+                     "frac_of_roof": (tg + tv) / t_all,
+                     "gemm_only_ms": t_g * 1e3, "gemm_tflops": g_flop / t_g / 1e12,
+                     "gemm_frac_of_roof": tg / t_g,
+                     "gemv_only_ms": t_v * 1e3, "gemv_gbps": float(v_bytes.sum()) / t_v / 1e9,
+                     "gemv_frac_of_hbm_peak": float(v_bytes.sum()) / t_v / PEAK_HBM,
+                     "gemm_tile_classes": classes,
+                     "roof_rate_tflops": rate / 1e12}
+
+    def per_instance():
+        for d in gemms:
+            m, n, k, a, lda, b, ldb, c, ldc, al, be = d
+            lib.omb_sgemm(m, n, k, al, a, lda, 1, b, ldb, 1, be, c, ldc, 1, pkg.PLAIN_F32, sp)
+        for d in gemvs:
+            m, n, a, lda, x, y, al, be = d
+            lib.omb_sgemv(m, n, al, a, lda, 1, x, be, y, sp)
+
+    per_instance()
+    out["per_instance_calls_ms"] = timer.run(per_instance, 2) * 1e3
+    out["speedup_vs_per_instance_calls"] = out["per_instance_calls_ms"] / out["plain_f32"]["ms"]
+    del pools
+    return out
+
+
+def serving_leg(pkg, torch, dev, stream, rate, count, slots):
+    """Serving loop: a Poisson trace (seed 4) of `count` requests at `rate`
+    per second, 50% GEMM (M, N, K log-uniform on [32, 4096], split-BF16),
+    30% GEMV (rows, cols log-uniform on [256, 16384]), 20% dot (n
+    log-uniform on [2^10, 2^24]), operands U[-1,1) resident in HBM,
+    admission by free HBM.  Reports arrival-to-result latency percentiles
+    and aggregate TFLOP/s = total work / wall; the same trace replayed with
+    one slot (FIFO, no overlap) is the comparison."""
+    log(f"serving: {count} requests at {rate:.0f}/s, {slots} slots (seed 4)")
+    sp = stream.cuda_stream
+    trace = pkg.serving_trace(count, rate, seed=4)
+    sz = []
+    for k, m, n, kk, _ in trace:
+        sz.append((m * kk, kk * n, m * n) if k == pkg.REQ_GEMM else
+                  ((m * n, n, m) if k == pkg.REQ_GEMV else (n, n, 1)))
+    sz = np.asarray(sz, dtype=np.int64)
+    pools = [_pool(torch, dev, stream, sz[:, j]) for j in range(3)]
+    pkg.fill_uniform(pools[0][0], 4, 0, stream=sp)
+    pkg.fill_uniform(pools[1][0], 4, 1, stream=sp)
+    torch.cuda.synchronize()
+
+    def requests():
+        return [pkg.Request(k, pkg.SPLIT_BF16, m, n, kk, t,
+                            *[pools[j][0].data_ptr() + 4 * int(pools[j][1][i]) for j in range(3)], 0)
+                for i, (k, m, n, kk, t) in enumerate(trace)]
+
+    pkg.serve_run(requests()[: min(64, count)], slots=slots)          # warm-up
+    res, st = pkg.serve_run(requests(), slots=slots)
+    kinds = np.array([t[0] for t in trace])
+    lat = np.array([r["latency_s"] for r in res])
+    svc = np.array([r["done_s"] - r["start_s"] for r in res])
+    ok = np.array([r["status"] == 0 for r in res])
+    busy_bytes = sum(pkg.request_bytes(k, m, n, kk) for k, m, n, kk, _ in trace)
+    out = {"rate_per_s": rate, "requests": count, "slots": slots, "completed": int(ok.sum()),
+           "p50_ms": st["p50_s"] * 1e3, "p90_ms": st["p90_s"] * 1e3, "p99_ms": st["p99_s"] * 1e3,
+           "max_ms": st["max_s"] * 1e3, "mean_ms": st["mean_s"] * 1e3,
+           "wall_s": st["wall_s"], "aggregate_tflops": st["total_flop"] / st["wall_s"] / 1e12,
+           "offered_tflops": st["total_flop"] / (trace[-1][4] or 1.0) / 1e12,
+           "operand_gbps": busy_bytes / st["wall_s"] / 1e9,
+           "max_in_flight": st["max_in_flight"], "admitted_late": st["admitted_late"],
+           "p99_ms_by_kind": {name: float(np.percentile(lat[kinds == k], 99)) * 1e3
+                              for k, name in ((0, "gemm"), (1, "gemv"), (2, "dot"))
+                              if (kinds == k).any()},
+           "service_p99_ms": float(np.percentile(svc, 99)) * 1e3}
+    sweep = []
+    for r in (8000.0, 16000.0, 48000.0):
+        tr = pkg.serving_trace(count, r, seed=4)
+        reqs = requests()
+        for q, t in zip(reqs, tr):
+            q.arrival_s = t[4]
+        _, s2 = pkg.serve_run(reqs, slots=slots)
+        sweep.append({"rate_per_s": r, "p50_ms": s2["p50_s"] * 1e3, "p99_ms": s2["p99_s"] * 1e3,
+                      "aggregate_tflops": s2["total_flop"] / s2["wall_s"] / 1e12})
+    out["rate_sweep"] = sweep
+    res1, st1 = pkg.serve_run(requests(), slots=1)
+    out["fifo_one_slot"] = {"p50_ms": st1["p50_s"] * 1e3, "p99_ms": st1["p99_s"] * 1e3,
+                            "aggregate_tflops": st1["total_flop"] / st1["wall_s"] / 1e12}
+    del pools
     return out
 
 

@@ -170,7 +170,12 @@ omb_status omb_autotune_load(const char* table_path);
  * resident operands.  Requests are replayed from a trace of arrival times;
  * each request is admitted when its HBM need fits the budget and a slot
  * (stream) is free, runs concurrently with requests already on the GPU, and
- * its completion time is recorded.  No counterpart in the reference. */
+ * its completion time is recorded.  The call first waits for all queued
+ * device work (so operands written on any stream are ready) and returns when
+ * every admitted request has finished.  A request whose workspace exceeds
+ * the budget is refused with OMB_EBUSY in its result; the others still run.
+ * Each result is bitwise the same as the request's solo run through
+ * omb_sgemm / omb_sgemv / omb_sdot.  No counterpart in the reference. */
 #define OMB_REQ_GEMM 0
 #define OMB_REQ_GEMV 1
 #define OMB_REQ_DOT 2

@@ -376,3 +376,71 @@ def autotune_group(gemms, mode: int = PLAIN_F32, table_path: Optional[str] = Non
 
 def autotune_load(table_path: str) -> None:
     check(lib().omb_autotune_load(table_path.encode()))
+
+
+# ---------------------------------------------------------- serving loop --
+def serve_run(requests, slots: int = 8, hbm_budget: int = 0, hbm_reserve: int = 1 << 30,
+              spin_poll: bool = True):
+    """Replay `requests` (a list of Request) through omb_serve_run.
+
+    Returns (results, stats): one dict per request (start_s, done_s,
+    latency_s, status, slot, ws_bytes) and a dict of the run's statistics.
+    A request whose workspace can never fit the budget comes back with
+    status EBUSY; any other failure raises OmbError."""
+    n = len(requests)
+    reqs = (Request * max(1, n))(*requests)
+    res = (RequestResult * max(1, n))()
+    st = ServeStats()
+    cfg = ServeConfig(slots, hbm_budget, hbm_reserve, 1 if spin_poll else 0)
+    check(lib().omb_serve_run(reqs, n, ctypes.byref(cfg), res, ctypes.byref(st)))
+    out = []
+    for i in range(n):
+        r = res[i]
+        out.append(dict(start_s=r.start_s, done_s=r.done_s, latency_s=r.latency_s,
+                        status=r.status, slot=r.slot, ws_bytes=reqs[i].ws_bytes))
+    stats = {name: getattr(st, name) for name, _ in ServeStats._fields_}
+    return out, stats
+
+
+def log_uniform_ints(rng, lo: int, hi: int, size):
+    """Integers log-uniform on [lo, hi]: floor(exp(U[ln lo, ln(hi + 1))))."""
+    import numpy as np
+    v = np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi + 1), size))).astype(np.int64)
+    return np.clip(v, lo, hi)
+
+
+def serving_trace(count: int, rate_hz: float, seed: int = 4, mix=(0.5, 0.3, 0.2),
+                  gemm=(32, 4096), gemv=(256, 16384), dot=(1 << 10, 1 << 24)):
+    """Poisson arrivals at `rate_hz` with the GEMM / GEMV / dot mix: GEMM
+    M, N, K, GEMV rows and cols, and dot length each log-uniform on the given
+    ranges.  Returns a list of (kind, m, n, k, arrival_s)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate_hz, count)
+    arrival = np.cumsum(gaps) - gaps[0]
+    kinds = rng.choice(3, size=count, p=np.asarray(mix, dtype=np.float64) / sum(mix))
+    g = log_uniform_ints(rng, gemm[0], gemm[1], (count, 3))
+    v = log_uniform_ints(rng, gemv[0], gemv[1], (count, 2))
+    d = log_uniform_ints(rng, dot[0], dot[1], count)
+    out = []
+    for i in range(count):
+        k = int(kinds[i])
+        if k == REQ_GEMM:
+            out.append((k, int(g[i, 0]), int(g[i, 1]), int(g[i, 2]), float(arrival[i])))
+        elif k == REQ_GEMV:
+            out.append((k, int(v[i, 0]), int(v[i, 1]), 0, float(arrival[i])))
+        else:
+            out.append((k, 0, int(d[i]), 0, float(arrival[i])))
+    return out
+
+
+def request_flop(kind: int, m: int, n: int, k: int) -> float:
+    return 2.0 * m * n * k if kind == REQ_GEMM else (2.0 * m * n if kind == REQ_GEMV else 2.0 * n)
+
+
+def request_bytes(kind: int, m: int, n: int, k: int) -> float:
+    if kind == REQ_GEMM:
+        return 4.0 * (m * k + k * n + m * n)
+    if kind == REQ_GEMV:
+        return 4.0 * (m * n + m + n)
+    return 8.0 * n

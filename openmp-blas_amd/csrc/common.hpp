@@ -90,6 +90,10 @@ omb_status launch_sgemv_row(int64_t M, int64_t N, float alpha, const float* A, i
 omb_status launch_sgemv_col(int64_t M, int64_t N, float alpha, const float* A, int64_t lda,
                             const float* x, float beta, float* y, hipStream_t s);
 omb_status launch_sdot(int64_t n, const float* x, const float* y, float* result, hipStream_t s);
+// SDOT with a caller-supplied partial buffer of sdot_blocks(n) doubles.
+int64_t sdot_blocks(int64_t n);
+omb_status launch_sdot_ws(int64_t n, const float* x, const float* y, float* result,
+                          double* part, hipStream_t s);
 
 omb_status launch_fill_uniform(float* dst, int64_t n, uint64_t seed, uint64_t stream_id,
                                int64_t offset, hipStream_t s);

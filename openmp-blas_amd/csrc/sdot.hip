@@ -109,20 +109,22 @@ __global__ void zero_scalar_kernel(float* r) { *r = 0.f; }
 
 }  // namespace
 
-omb_status launch_sdot(int64_t n, const float* x, const float* y, float* result, hipStream_t s) {
+int64_t sdot_blocks(int64_t n) {
+    // ~8 float4 of each operand per thread at least; up to 8 workgroups per CU.
+    return std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, ceil_div(n, NT * 4 * 8)));
+}
+
+omb_status launch_sdot_ws(int64_t n, const float* x, const float* y, float* result,
+                          double* part, hipStream_t s) {
     if (n <= 0) {
         hipLaunchKernelGGL(zero_scalar_kernel, dim3(1), dim3(1), 0, s, result);
         OMB_HIP_TRY(hipGetLastError());
         return OMB_OK;
     }
-    // ~8 float4 of each operand per thread at least; up to 8 workgroups per CU.
-    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, ceil_div(n, NT * 4 * 8)));
-    Workspace* ws = nullptr;
-    OMB_TRY(get_workspace(s, (size_t)blocks * sizeof(double), &ws));
+    const int64_t blocks = sdot_blocks(n);
     const bool vec = ((reinterpret_cast<uintptr_t>(x) & 15) == 0) &&
                      ((reinterpret_cast<uintptr_t>(y) & 15) == 0);
     const int f64 = n >= kF64Combine ? 1 : 0;
-    double* part = reinterpret_cast<double*>(ws->buf);
     if (vec)
         hipLaunchKernelGGL(sdot_kernel<true>, dim3((unsigned)blocks), dim3(NT), 0, s, n, x, y,
                            result, part);
@@ -134,6 +136,12 @@ omb_status launch_sdot(int64_t n, const float* x, const float* y, float* result,
                            result);
     OMB_HIP_TRY(hipGetLastError());
     return OMB_OK;
+}
+
+omb_status launch_sdot(int64_t n, const float* x, const float* y, float* result, hipStream_t s) {
+    Workspace* ws = nullptr;
+    OMB_TRY(get_workspace(s, n > 0 ? (size_t)sdot_blocks(n) * sizeof(double) : 0, &ws));
+    return launch_sdot_ws(n, x, y, result, reinterpret_cast<double*>(ws->buf), s);
 }
 
 }  // namespace omb

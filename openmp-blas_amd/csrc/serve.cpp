@@ -1,0 +1,348 @@
+// Request-serving loop: continuous batching of mixed GEMM / GEMV / dot
+// requests over device-resident operands.  (The reference has no serving
+// layer; each request is one of its mtm / mtv / dot calls.)
+//
+// The loop replays a trace of arrival times on the calling thread:
+//   - a request becomes eligible at its arrival time;
+//   - it is admitted when a slot (one HIP stream each) is free and its
+//     workspace (split-BF16 planes, dot partials) fits in the HBM budget,
+//     taken first-fit from one pre-allocated arena -- eligible requests are
+//     scanned in arrival order and one that does not fit yet does not hold
+//     back smaller ones behind it;
+//   - admitted requests run concurrently with those already on the GPU;
+//   - completions are observed by polling one event per slot, which frees
+//     the slot and the workspace for the next admissions.
+// Kernels are the same deterministic ones as the direct entry points and
+// each request owns its workspace, so a result does not depend on what ran
+// beside it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <list>
+#include <thread>
+#include <vector>
+
+#include "common.hpp"
+
+namespace omb {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// First-fit allocator over one device buffer (256-byte granules).
+class Arena {
+   public:
+    ~Arena() {
+        if (base_) (void)hipFree(base_);
+    }
+    omb_status init(size_t bytes) {
+        size_ = round_up((int64_t)std::max<size_t>(bytes, 256), 256);
+        if (hipMalloc(&base_, size_) != hipSuccess) {
+            (void)hipGetLastError();
+            base_ = nullptr;
+            set_error("serve: arena allocation of " + std::to_string(size_) + " bytes failed");
+            return OMB_ENOMEM;
+        }
+        free_.clear();
+        free_.push_back({0, size_});
+        return OMB_OK;
+    }
+    size_t capacity() const { return size_; }
+    bool alloc(size_t bytes, size_t* off) {
+        bytes = round_up((int64_t)std::max<size_t>(bytes, 1), 256);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->size >= bytes) {
+                *off = it->off;
+                it->off += bytes;
+                it->size -= bytes;
+                if (it->size == 0) free_.erase(it);
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes) {
+        bytes = round_up((int64_t)std::max<size_t>(bytes, 1), 256);
+        auto it = free_.begin();
+        while (it != free_.end() && it->off < off) ++it;
+        it = free_.insert(it, {off, bytes});
+        auto next = std::next(it);
+        if (next != free_.end() && it->off + it->size == next->off) {
+            it->size += next->size;
+            free_.erase(next);
+        }
+        if (it != free_.begin()) {
+            auto prev = std::prev(it);
+            if (prev->off + prev->size == it->off) {
+                prev->size += it->size;
+                free_.erase(it);
+            }
+        }
+    }
+    char* base() const { return static_cast<char*>(base_); }
+
+   private:
+    struct Block {
+        size_t off, size;
+    };
+    void* base_ = nullptr;
+    size_t size_ = 0;
+    std::list<Block> free_;
+};
+
+size_t need_bytes(const omb_request& r) {
+    switch (r.kind) {
+        case OMB_REQ_GEMM:
+            if (r.mode == OMB_MODE_PLAIN_F32 || r.m == 0 || r.n == 0 || r.k == 0) return 0;
+            return (size_t)(split_packed_elems(r.m, r.k) + split_packed_elems(r.n, r.k)) * 2;
+        case OMB_REQ_DOT: return r.n > 0 ? (size_t)sdot_blocks(r.n) * sizeof(double) : 0;
+        default: return 0;
+    }
+}
+
+double flop_of(const omb_request& r) {
+    switch (r.kind) {
+        case OMB_REQ_GEMM: return 2.0 * r.m * r.n * r.k;
+        case OMB_REQ_GEMV: return 2.0 * r.m * r.n;
+        default: return 2.0 * r.n;
+    }
+}
+
+omb_status check_request(const omb_request& r, int i) {
+    bool ok = true;
+    switch (r.kind) {
+        case OMB_REQ_GEMM:
+            ok = r.m >= 0 && r.n >= 0 && r.k >= 0 &&
+                 (r.mode == OMB_MODE_PLAIN_F32 || r.mode == OMB_MODE_SPLIT_BF16 ||
+                  r.mode == OMB_MODE_SPLIT_BF16_P3) &&
+                 (r.m == 0 || r.n == 0 || (r.c && (r.k == 0 || (r.a && r.b))));
+            break;
+        case OMB_REQ_GEMV:
+            ok = r.m >= 0 && r.n >= 0 && (r.m == 0 || (r.c && (r.n == 0 || (r.a && r.b))));
+            break;
+        case OMB_REQ_DOT: ok = r.n >= 0 && r.c && (r.n == 0 || (r.a && r.b)); break;
+        default: ok = false;
+    }
+    if (!ok || !(r.arrival_s >= 0.0) || !std::isfinite(r.arrival_s)) {
+        set_error("omb_serve_run: invalid request " + std::to_string(i));
+        return OMB_EINVAL;
+    }
+    return OMB_OK;
+}
+
+// One request on stream s; all operands row-major and contiguous.
+omb_status launch_request(const omb_request& r, char* ws, hipStream_t s) {
+    switch (r.kind) {
+        case OMB_REQ_GEMM: {
+            if (r.m == 0 || r.n == 0) return OMB_OK;
+            if (r.k == 0) return launch_scale(r.m, r.n, 0.f, r.c, r.n, s);
+            if (r.mode == OMB_MODE_PLAIN_F32)
+                return launch_sgemm_f32(r.m, r.n, r.k, 1.f, r.a, r.k, true, r.b, r.n, true, 0.f, r.c,
+                                        r.n, s);
+            __bf16* ap = reinterpret_cast<__bf16*>(ws);
+            __bf16* bp = ap + split_packed_elems(r.m, r.k);
+            OMB_TRY(launch_split_pack(r.m, r.k, r.a, r.k, 1, ap, s));
+            OMB_TRY(launch_split_pack(r.n, r.k, r.b, 1, r.n, bp, s));  // B^T of row-major B
+            return launch_sgemm_packed(r.m, r.n, r.k, 1.f, ap, bp, 0.f, r.c, r.n,
+                                       r.mode == OMB_MODE_SPLIT_BF16_P3 ? 3 : 6, s);
+        }
+        case OMB_REQ_GEMV:
+            if (r.m == 0) return OMB_OK;
+            if (r.n == 0) return launch_scale(r.m, 1, 0.f, r.c, 1, s);
+            return launch_sgemv_row(r.m, r.n, 1.f, r.a, r.n, r.b, 0.f, r.c, s);
+        default:
+            return launch_sdot_ws(r.n, r.a, r.b, r.c, reinterpret_cast<double*>(ws), s);
+    }
+}
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    int req = -1;
+    size_t off = 0, bytes = 0;
+};
+
+double pct(const std::vector<double>& sorted, double q) {
+    if (sorted.empty()) return 0.0;
+    size_t i = (size_t)std::ceil(q * (double)sorted.size());
+    i = i == 0 ? 0 : i - 1;
+    return sorted[std::min(i, sorted.size() - 1)];
+}
+
+}  // namespace
+}  // namespace omb
+
+using namespace omb;
+
+extern "C" omb_status omb_serve_run(omb_request* reqs, int32_t n, const omb_serve_config* cfg,
+                                    omb_request_result* results, omb_serve_stats* stats) {
+    clear_error();
+    if (n < 0 || (n > 0 && (!reqs || !results)) || !cfg || cfg->slots < 1 || cfg->slots > 64 ||
+        cfg->hbm_budget < 0 || cfg->hbm_reserve < 0) {
+        set_error("omb_serve_run: invalid arguments");
+        return OMB_EINVAL;
+    }
+    for (int i = 0; i < n; ++i) OMB_TRY(check_request(reqs[i], i));
+
+    std::vector<int> order(n);
+    size_t max_need = 0;
+    for (int i = 0; i < n; ++i) {
+        order[i] = i;
+        reqs[i].ws_bytes = (int64_t)need_bytes(reqs[i]);
+        max_need = std::max(max_need, (size_t)reqs[i].ws_bytes);
+        results[i] = omb_request_result{0.0, 0.0, 0.0, OMB_OK, -1};
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return reqs[a].arrival_s < reqs[b].arrival_s; });
+
+    size_t budget = (size_t)cfg->hbm_budget;
+    if (budget == 0) {
+        size_t freeb = 0, total = 0;
+        OMB_HIP_TRY(hipMemGetInfo(&freeb, &total));
+        budget = freeb > (size_t)cfg->hbm_reserve ? freeb - (size_t)cfg->hbm_reserve : 0;
+    }
+    // At most `slots` requests hold workspace at once.
+    const size_t arena_bytes = std::min(budget, (size_t)cfg->slots * std::max<size_t>(max_need, 256));
+    Arena arena;
+    if (n > 0) OMB_TRY(arena.init(arena_bytes));
+
+    std::vector<Slot> slots(cfg->slots);
+    auto cleanup = [&]() {
+        for (auto& s : slots) {
+            if (s.stream) (void)hipStreamSynchronize(s.stream);
+            if (s.done) (void)hipEventDestroy(s.done);
+            if (s.stream) (void)hipStreamDestroy(s.stream);
+        }
+    };
+    for (auto& s : slots) {
+        if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            cleanup();
+            set_error("omb_serve_run: stream/event creation failed");
+            return OMB_EDEVICE;
+        }
+    }
+    // Operands may still be in production on other streams (the slots do not
+    // wait for the null stream): finish all queued device work first.
+    if (hipDeviceSynchronize() != hipSuccess) {
+        (void)hipGetLastError();
+        cleanup();
+        set_error("omb_serve_run: device synchronisation failed");
+        return OMB_EDEVICE;
+    }
+
+    const auto t0 = Clock::now();
+    auto now_s = [&]() { return std::chrono::duration<double>(Clock::now() - t0).count(); };
+    std::list<int> waiting;
+    std::vector<char> deferred(n, 0);  // skipped by an admission scan (slot or memory)
+    int next = 0, finished = 0, in_flight = 0, max_in_flight = 0, late = 0;
+    double last_done = 0.0;
+    while (finished < n) {
+        bool progress = false;
+        double t = now_s();
+        // 1. completions
+        for (int si = 0; si < (int)slots.size(); ++si) {
+            Slot& s = slots[si];
+            if (s.req < 0) continue;
+            hipError_t q = hipEventQuery(s.done);
+            if (q == hipErrorNotReady) continue;
+            omb_request_result& res = results[s.req];
+            if (q != hipSuccess) {
+                (void)hipGetLastError();
+                res.status = OMB_EDEVICE;
+            }
+            t = now_s();
+            res.done_s = t;
+            res.latency_s = t - reqs[s.req].arrival_s;
+            last_done = t;
+            if (s.bytes) arena.release(s.off, s.bytes);
+            s.req = -1;
+            --in_flight;
+            ++finished;
+            progress = true;
+        }
+        // 2. arrivals
+        while (next < n && reqs[order[next]].arrival_s <= t) waiting.push_back(order[next++]);
+        // 3. admissions, in arrival order
+        for (auto it = waiting.begin(); it != waiting.end() && in_flight < (int)slots.size();) {
+            const int r = *it;
+            const size_t need = (size_t)reqs[r].ws_bytes;
+            if (need > arena.capacity()) {  // can never fit: refuse instead of waiting forever
+                results[r].status = OMB_EBUSY;
+                results[r].start_s = results[r].done_s = t;
+                results[r].latency_s = t - reqs[r].arrival_s;
+                ++finished;
+                it = waiting.erase(it);
+                progress = true;
+                continue;
+            }
+            size_t off = 0;
+            if (need > 0 && !arena.alloc(need, &off)) {
+                ++it;
+                continue;
+            }
+            int si = 0;
+            while (slots[si].req >= 0) ++si;
+            Slot& s = slots[si];
+            omb_status st = launch_request(reqs[r], need ? arena.base() + off : nullptr, s.stream);
+            if (st == OMB_OK && hipEventRecord(s.done, s.stream) != hipSuccess) st = OMB_EDEVICE;
+            const double ts = now_s();
+            results[r].start_s = ts;
+            results[r].slot = si;
+            if (deferred[r]) ++late;
+            if (st != OMB_OK) {
+                (void)hipGetLastError();
+                results[r].status = st;
+                results[r].done_s = ts;
+                results[r].latency_s = ts - reqs[r].arrival_s;
+                if (need) arena.release(off, need);
+                ++finished;
+            } else {
+                s.req = r;
+                s.off = off;
+                s.bytes = need;
+                ++in_flight;
+                max_in_flight = std::max(max_in_flight, in_flight);
+            }
+            it = waiting.erase(it);
+            progress = true;
+        }
+        for (int r : waiting) deferred[r] = 1;
+        if (!progress) {
+            if (in_flight == 0 && waiting.empty() && next < n) {
+                // idle until the next arrival: sleep most of the gap, spin the rest
+                const double gap = reqs[order[next]].arrival_s - now_s();
+                if (gap > 200e-6 && !cfg->spin_poll)
+                    std::this_thread::sleep_for(std::chrono::duration<double>(gap - 100e-6));
+            } else if (!cfg->spin_poll) {
+                std::this_thread::yield();
+            }
+        }
+    }
+    cleanup();
+
+    if (stats) {
+        std::vector<double> lat;
+        double flop = 0.0, sum = 0.0;
+        for (int i = 0; i < n; ++i) {
+            if (results[i].status != OMB_OK) continue;
+            lat.push_back(results[i].latency_s);
+            sum += results[i].latency_s;
+            flop += flop_of(reqs[i]);
+        }
+        std::sort(lat.begin(), lat.end());
+        stats->wall_s = last_done;
+        stats->p50_s = pct(lat, 0.50);
+        stats->p90_s = pct(lat, 0.90);
+        stats->p99_s = pct(lat, 0.99);
+        stats->max_s = lat.empty() ? 0.0 : lat.back();
+        stats->mean_s = lat.empty() ? 0.0 : sum / (double)lat.size();
+        stats->total_flop = flop;
+        stats->max_in_flight = max_in_flight;
+        stats->admitted_late = late;
+    }
+    return OMB_OK;
+}

@@ -136,7 +136,6 @@ struct MArgs {
 
 constexpr int VA = SLICE_A / 8 / NT;  // 16-byte loads per thread per slice (3)
 constexpr int VB = SLICE_B / 8 / NT;
-constexpr int V = VA + VB;
 
 // One slice in flight is 6 uint4 per thread (3 of A, 3 of B), kept in named
 // registers (arrays of them were demoted to scratch by the compiler).

@@ -114,3 +114,52 @@ def test_group_plan_rejects_bad_descriptors(ompblas):
     assert lib.omb_group_plan_create(None, -1, None, 0, 0, None, ctypes.byref(h)) == -1
     assert "group plan" in ompblas.last_error()
     assert lib.omb_autotune_load(b"/nonexistent/table.txt") == -1
+
+
+def test_serve_run_rejects_bad_arguments(ompblas):
+    """Trace and configuration validation runs before any device call."""
+    import ctypes
+    lib = ompblas.lib()
+    R, Cfg = ompblas.Request, ompblas.ServeConfig
+    res = (ompblas.RequestResult * 2)()
+    st = ompblas.ServeStats()
+    good = Cfg(2, 0, 0, 1)
+
+    def run(reqs, cfg=good, n=None):
+        arr = (R * max(1, len(reqs)))(*reqs)
+        return lib.omb_serve_run(arr, len(reqs) if n is None else n, ctypes.byref(cfg), res,
+                                 ctypes.byref(st))
+
+    ok = R(ompblas.REQ_DOT, 0, 0, 8, 0, 0.0, 16, 16, 16, 0)
+    assert run([ok], n=-1) == -1
+    assert run([ok], cfg=Cfg(0, 0, 0, 1)) == -1                      # no slots
+    assert run([ok], cfg=Cfg(2, -5, 0, 1)) == -1                     # negative budget
+    assert run([R(7, 0, 1, 1, 1, 0.0, 16, 16, 16, 0)]) == -1         # unknown kind
+    assert run([R(ompblas.REQ_GEMM, 9, 4, 4, 4, 0.0, 16, 16, 16, 0)]) == -1   # bad mode
+    assert run([R(ompblas.REQ_GEMM, 1, 4, 4, 4, 0.0, 16, None, 16, 0)]) == -1  # missing B
+    assert run([R(ompblas.REQ_GEMV, 0, 4, 4, 0, 0.0, 16, 16, None, 0)]) == -1  # missing y
+    assert run([ok, R(ompblas.REQ_DOT, 0, 0, 8, 0, -1.0, 16, 16, 16, 0)]) == -1  # arrival < 0
+    assert run([R(ompblas.REQ_DOT, 0, 0, 8, 0, float("nan"), 16, 16, 16, 0)]) == -1
+    assert "omb_serve_run" in ompblas.last_error()
+
+
+def test_serving_trace_mix_and_ranges(ompblas):
+    """The Poisson trace: seeded, sorted arrivals, the 50/30/20 mix and the
+    log-uniform size ranges."""
+    import numpy as np
+    t = ompblas.serving_trace(20000, 1000.0, seed=4)
+    assert t == ompblas.serving_trace(20000, 1000.0, seed=4)
+    kinds = np.array([x[0] for x in t])
+    frac = [(kinds == k).mean() for k in range(3)]
+    assert abs(frac[0] - 0.5) < 0.02 and abs(frac[1] - 0.3) < 0.02 and abs(frac[2] - 0.2) < 0.02
+    arr = np.array([x[4] for x in t])
+    assert arr[0] == 0.0 and np.all(np.diff(arr) >= 0)
+    assert abs(np.diff(arr).mean() - 1e-3) < 5e-5                   # rate 1000/s
+    g = np.array([x[1:4] for x in t if x[0] == 0])
+    assert g.min() >= 32 and g.max() <= 4096
+    # log-uniform: the median of each dimension sits near the geometric mean
+    assert abs(np.median(g) - np.sqrt(32 * 4097)) / np.sqrt(32 * 4097) < 0.1
+    v = np.array([x[1:3] for x in t if x[0] == 1])
+    assert v.min() >= 256 and v.max() <= 16384
+    d = np.array([x[2] for x in t if x[0] == 2])
+    assert d.min() >= 1024 and d.max() <= 1 << 24

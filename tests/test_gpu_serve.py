@@ -1,0 +1,176 @@
+"""Serving loop (omb_serve_run): mixed GEMM / GEMV / dot requests admitted by
+HBM budget and overlapped on several streams.  Every result must be bitwise
+the same as the request's solo run through the direct entry points, whatever
+ran beside it; a request whose workspace can never fit is refused."""
+import numpy as np
+import pytest
+import torch
+
+import helpers
+
+pytestmark = pytest.mark.gpu
+
+
+class Trace:
+    """Device-resident operands for a serving trace (row-major, one
+    allocation per operand so each request's buffers are independent)."""
+
+    def __init__(self, dev, ompblas, trace, seed, mode):
+        self.items = []
+        for i, (kind, m, n, k, t) in enumerate(trace):
+            if kind == ompblas.REQ_GEMM:
+                sa, sb, sc = m * k, k * n, m * n
+            elif kind == ompblas.REQ_GEMV:
+                sa, sb, sc = m * n, n, m
+            else:
+                sa, sb, sc = n, n, 1
+            a = torch.empty(max(sa, 1), device=dev)
+            b = torch.empty(max(sb, 1), device=dev)
+            c = torch.full((max(sc, 1),), float("nan"), device=dev)
+            ompblas.fill_uniform(a, seed, 2 * i)
+            ompblas.fill_uniform(b, seed, 2 * i + 1)
+            self.items.append((kind, m, n, k, t, a, b, c))
+        self.mode = mode
+        self.ompblas = ompblas
+
+    def requests(self, subset=None):
+        R = self.ompblas.Request
+        idx = range(len(self.items)) if subset is None else subset
+        out = []
+        for i in idx:
+            kind, m, n, k, t, a, b, c = self.items[i]
+            out.append(R(kind, self.mode, m, n, k, t if subset is None else 0.0,
+                         a.data_ptr(), b.data_ptr(), c.data_ptr(), 0))
+        return out
+
+    def outputs(self):
+        return [it[7].clone() for it in self.items]
+
+    def clear(self):
+        for it in self.items:
+            it[7].fill_(float("nan"))
+
+    def solo(self, i):
+        """Request i through the direct entry point (C = A*B, y = A*x, r = x.y)."""
+        p = self.ompblas
+        kind, m, n, k, t, a, b, c = self.items[i]
+        out = torch.full_like(c, float("nan"))
+        if kind == p.REQ_GEMM:
+            p.check(p.sgemm(m, n, k, 1.0, a.data_ptr(), k, 1, b.data_ptr(), n, 1, 0.0,
+                            out.data_ptr(), n, 1, self.mode))
+        elif kind == p.REQ_GEMV:
+            p.check(p.sgemv(m, n, 1.0, a.data_ptr(), n, 1, b.data_ptr(), 0.0, out.data_ptr()))
+        else:
+            p.check(p.sdot(n, a.data_ptr(), b.data_ptr(), out.data_ptr()))
+        return out
+
+
+def small_trace(ompblas, count, rate, seed):
+    return ompblas.serving_trace(count, rate, seed=seed, gemm=(32, 768), gemv=(256, 4096),
+                                 dot=(1 << 10, 1 << 20))
+
+
+@pytest.mark.parametrize("mode", [1, 0])
+def test_mix_equals_solo_bitwise(dev, ompblas, mode):
+    """High arrival rate, so requests overlap on the device; each output is
+    bitwise equal to its solo run."""
+    trace = small_trace(ompblas, 160, 2.0e5, seed=4)
+    tr = Trace(dev, ompblas, trace, seed=4, mode=mode)
+    res, st = ompblas.serve_run(tr.requests(), slots=8)
+    torch.cuda.synchronize()
+    assert all(r["status"] == 0 for r in res)
+    assert st["max_in_flight"] > 1, st
+    mixed = tr.outputs()
+    for i in range(len(trace)):
+        solo = tr.solo(i)
+        assert not torch.isnan(mixed[i]).any(), i
+        assert torch.equal(mixed[i], solo), (i, trace[i])
+    # statistics are consistent with the per-request records
+    lat = sorted(r["latency_s"] for r in res)
+    assert st["p50_s"] <= st["p99_s"] <= st["max_s"] == lat[-1]
+    assert all(r["done_s"] >= r["start_s"] >= 0 for r in res)
+    assert all(r["start_s"] >= t[4] for r, t in zip(res, trace))
+    want_flop = sum(ompblas.request_flop(k, m, n, kk) for k, m, n, kk, _ in trace)
+    assert st["total_flop"] == pytest.approx(want_flop, rel=1e-12)
+
+
+def test_solo_run_through_the_loop_matches(dev, ompblas):
+    """A request run alone through the loop equals the same request in the mix."""
+    trace = small_trace(ompblas, 40, 1.0e5, seed=14)
+    tr = Trace(dev, ompblas, trace, seed=14, mode=1)
+    ompblas.serve_run(tr.requests(), slots=6)
+    torch.cuda.synchronize()
+    mixed = tr.outputs()
+    tr.clear()
+    for i in range(len(trace)):
+        res, _ = ompblas.serve_run(tr.requests([i]), slots=1)
+        assert res[0]["status"] == 0
+    torch.cuda.synchronize()
+    alone = tr.outputs()
+    for i in range(len(trace)):
+        assert torch.equal(mixed[i], alone[i]), (i, trace[i])
+
+
+def test_small_budget_makes_requests_wait(dev, ompblas):
+    """A budget of one split GEMM's workspace serialises the GEMMs; they
+    wait for memory instead of failing, and results are unchanged."""
+    trace = [(ompblas.REQ_GEMM, 300 + 7 * i, 200 + 5 * i, 260 + 3 * i, 0.0) for i in range(12)]
+    trace += [(ompblas.REQ_DOT, 0, 5000 + i, 0, 0.0) for i in range(4)]
+    tr = Trace(dev, ompblas, trace, seed=24, mode=1)
+    need = max(ompblas.lib().omb_split_pack_bytes(m, k) + ompblas.lib().omb_split_pack_bytes(n, k)
+               for _, m, n, k, _ in trace[:12])
+    res, st = ompblas.serve_run(tr.requests(), slots=8, hbm_budget=need + 4096)
+    torch.cuda.synchronize()
+    assert all(r["status"] == 0 for r in res)
+    assert st["admitted_late"] > 0
+    mixed = tr.outputs()
+    for i in range(len(trace)):
+        assert torch.equal(mixed[i], tr.solo(i)), i
+
+
+def test_oversize_request_is_refused(dev, ompblas):
+    """A request whose workspace exceeds the budget gets EBUSY; the rest run."""
+    trace = [(ompblas.REQ_GEMM, 512, 512, 512, 0.0), (ompblas.REQ_GEMV, 300, 400, 0, 0.0),
+             (ompblas.REQ_DOT, 0, 1000, 0, 0.0)]
+    tr = Trace(dev, ompblas, trace, seed=34, mode=1)
+    res, st = ompblas.serve_run(tr.requests(), slots=2, hbm_budget=64 * 1024)
+    torch.cuda.synchronize()
+    assert res[0]["status"] == ompblas.EBUSY
+    assert res[0]["ws_bytes"] > 64 * 1024
+    assert res[1]["status"] == 0 and res[2]["status"] == 0
+    out = tr.outputs()
+    assert torch.equal(out[1], tr.solo(1)) and torch.equal(out[2], tr.solo(2))
+    assert torch.isnan(out[0]).all()
+
+
+def test_split_gemm_requests_within_error_bound(dev, ompblas):
+    """Split-BF16 requests keep the <= 2x plain FP32 error bound vs FP64."""
+    trace = [(ompblas.REQ_GEMM, 777, 640, 1500, 0.0), (ompblas.REQ_GEMM, 64, 2048, 4096, 0.0)]
+    tr = Trace(dev, ompblas, trace, seed=44, mode=1)
+    ompblas.serve_run(tr.requests(), slots=2)
+    torch.cuda.synchronize()
+    for i, (_, m, n, k, _) in enumerate(trace):
+        _, _, _, _, _, a, b, c = tr.items[i]
+        A, B = a.view(m, k), b.view(k, n)
+        c64 = ompblas.check_dgemm(A, B)
+        split = ompblas.rel_errors(c.view(m, n), c64)
+        plain = torch.zeros(m, n, device=dev)
+        ompblas.mtm(plain, A, B)()
+        ref = ompblas.rel_errors(plain, c64)
+        assert split[0] <= 2 * ref[0] and split[1] <= 2 * ref[1], (split, ref)
+
+
+def test_empty_and_degenerate_requests(dev, ompblas):
+    trace = [(ompblas.REQ_GEMM, 5, 7, 0, 0.0), (ompblas.REQ_GEMV, 9, 0, 0, 0.0),
+             (ompblas.REQ_DOT, 0, 0, 0, 0.0), (ompblas.REQ_GEMM, 0, 4, 4, 0.0)]
+    tr = Trace(dev, ompblas, trace, seed=54, mode=1)
+    res, _ = ompblas.serve_run(tr.requests(), slots=2)
+    torch.cuda.synchronize()
+    assert all(r["status"] == 0 for r in res)
+    out = tr.outputs()
+    assert torch.equal(out[0], torch.zeros(35, device=dev))
+    assert torch.equal(out[1], torch.zeros(9, device=dev))
+    assert torch.equal(out[2], torch.zeros(1, device=dev))
+    assert torch.isnan(out[3]).all()      # M == 0: nothing written
+    res, st = ompblas.serve_run([], slots=2)
+    assert res == [] and st["p99_s"] == 0.0
