@@ -65,6 +65,9 @@ def main():
     ap.add_argument("--no-errors", action="store_true", help="skip the FP64 error legs")
     ap.add_argument("--cpu-budget", type=float, default=25.0,
                     help="seconds of host time for the CPU baseline sample")
+    ap.add_argument("--legs", default="all",
+                    help="comma list of legs beside the headline: plain, errors, gemv, dot, "
+                         "grouped, serving, cpu (default all)")
     ap.add_argument("--serve-rate", type=float, default=float(os.environ.get("OMB_SERVE_RATE", 32000)),
                     help="serving leg: Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-count", type=int, default=int(os.environ.get("OMB_SERVE_COUNT", 3000)),
@@ -76,6 +79,14 @@ def main():
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is a design, not a measured path)", file=sys.stderr)
         sys.exit(2)
+
+    legs = {"plain", "errors", "gemv", "dot", "grouped", "serving", "cpu"}
+    if args.legs != "all":
+        legs = {x.strip() for x in args.legs.split(",") if x.strip()}
+    if args.no_errors:
+        legs.discard("errors")
+    if args.no_cpu:
+        legs.discard("cpu")
 
     import torch
 
@@ -121,6 +132,12 @@ def main():
     wall = time.perf_counter() - t0
     value = flop / t_step / 1e12
 
+    # a plain copy of known size (256 MiB read + 256 MiB written): the
+    # reference point that scales the profiler's FETCH_SIZE / WRITE_SIZE
+    with torch.cuda.stream(stream):
+        torch.mul(A, 1.0, out=C)
+    sync()
+
     # the MFMA kernel alone (pre-packed operands), for the roofline
     pbytes = lib.omb_split_pack_bytes(n, n)
     with torch.cuda.stream(stream):
@@ -151,73 +168,82 @@ def main():
     t_step3 = timer.run(lambda: gemm(pkg.SPLIT_BF16_P3), max(3, args.steps // 2))
 
     # --------------------------------------------------------- plain FP32 --
-    log("plain fp32 sgemm 8192^3 (seed 1, row and column major)")
-    pkg.fill_uniform(A, 1, 0, stream=sp)
-    pkg.fill_uniform(B, 1, 1, stream=sp)
     plain = {}
-    for layout in ("row", "col"):
+    for layout in (("row", "col") if "plain" in legs else ()):
+        if layout == "row":
+            log("plain fp32 sgemm 8192^3 (seed 1, row and column major)")
+            pkg.fill_uniform(A, 1, 0, stream=sp)
+            pkg.fill_uniform(B, 1, 1, stream=sp)
         fn = lambda: gemm(pkg.PLAIN_F32, layout=layout)
         fn()
         t_p = timer.run(fn, max(3, args.steps // 2))
         plain[layout] = {"ms": t_p * 1e3, "tflops": flop / t_p / 1e12,
                          "frac_of_fp32_peak": flop / t_p / PEAK_FP32}
-    kernels["sgemm_f32_kernel (8192^3 row-major)"] = dict(plain["row"])
+    if plain:
+        kernels["sgemm_f32_kernel (8192^3 row-major)"] = dict(plain["row"])
 
     # ------------------------------------------------------------- errors --
     errors = {}
-    if not args.no_errors:
+    if "errors" in legs:
         errors = error_leg(pkg, lib, torch, dev, sp, A, B, C)
     sync()
 
     # --------------------------------------------------------- GEMV / dot --
-    log("sgemv 16384^2 and sdot 2^28")
     del C
-    m = 16384
-    with torch.cuda.stream(stream):
-        GA = torch.empty((m, m), device=dev)
-        gx = torch.empty(m, device=dev)
-        gy = torch.zeros(m, device=dev)
-    pkg.fill_uniform(GA, 1, 0, stream=sp)
-    pkg.fill_uniform(gx, 1, 1, stream=sp)
-    gemv = {}
-    for layout, (rs, cs, beta) in (("row", (m, 1, 0.0)), ("col", (1, m, 1.0))):
-        fn = lambda: lib.omb_sgemv(m, m, 1.0, GA.data_ptr(), rs, cs, gx.data_ptr(), beta,
-                                   gy.data_ptr(), sp)
+    gemv, dot = {}, {}
+    if "gemv" in legs:
+        log("sgemv 16384^2 (seed 1, row and column major)")
+        m = 16384
+        with torch.cuda.stream(stream):
+            GA = torch.empty((m, m), device=dev)
+            gx = torch.empty(m, device=dev)
+            gy = torch.zeros(m, device=dev)
+        pkg.fill_uniform(GA, 1, 0, stream=sp)
+        pkg.fill_uniform(gx, 1, 1, stream=sp)
+        for layout, (rs, cs, beta) in (("row", (m, 1, 0.0)), ("col", (1, m, 1.0))):
+            fn = lambda: lib.omb_sgemv(m, m, 1.0, GA.data_ptr(), rs, cs, gx.data_ptr(), beta,
+                                       gy.data_ptr(), sp)
+            for _ in range(3):
+                fn()
+            t_g = timer.run(fn, max(10, args.steps))
+            nbytes = 4.0 * (m * m + m + m) + (4.0 * m if beta != 0.0 else 0.0)
+            gemv[layout] = {"us": t_g * 1e6, "gbps": nbytes / t_g / 1e9,
+                            "frac_of_hbm_peak": nbytes / t_g / PEAK_HBM, "bytes": nbytes}
+        kernels["sgemv_row_kernel (16384^2)"] = gemv["row"]
+        kernels["sgemv_col_kernel (16384^2)"] = gemv["col"]
+        del GA
+    if "dot" in legs:
+        log("sdot 2^28 (seed 1)")
+        nd = 1 << 28
+        with torch.cuda.stream(stream):
+            dx = torch.empty(nd, device=dev)
+            dy = torch.empty(nd, device=dev)
+            dr = torch.zeros(1, device=dev)
+        pkg.fill_uniform(dx, 1, 0, stream=sp)
+        pkg.fill_uniform(dy, 1, 1, stream=sp)
+        fn = lambda: lib.omb_sdot(nd, dx.data_ptr(), dy.data_ptr(), dr.data_ptr(), sp)
         for _ in range(3):
             fn()
-        t_g = timer.run(fn, max(10, args.steps))
-        nbytes = 4.0 * (m * m + m + m) + (4.0 * m if beta != 0.0 else 0.0)
-        gemv[layout] = {"us": t_g * 1e6, "gbps": nbytes / t_g / 1e9,
-                        "frac_of_hbm_peak": nbytes / t_g / PEAK_HBM, "bytes": nbytes}
-    kernels["sgemv_row_kernel (16384^2)"] = gemv["row"]
-    kernels["sgemv_col_kernel (16384^2)"] = gemv["col"]
-    del GA
-    nd = 1 << 28
-    with torch.cuda.stream(stream):
-        dx = torch.empty(nd, device=dev)
-        dy = torch.empty(nd, device=dev)
-        dr = torch.zeros(1, device=dev)
-    pkg.fill_uniform(dx, 1, 0, stream=sp)
-    pkg.fill_uniform(dy, 1, 1, stream=sp)
-    fn = lambda: lib.omb_sdot(nd, dx.data_ptr(), dy.data_ptr(), dr.data_ptr(), sp)
-    for _ in range(3):
-        fn()
-    t_d = timer.run(fn, max(10, args.steps))
-    dot = {"us": t_d * 1e6, "gbps": 8.0 * nd / t_d / 1e9, "frac_of_hbm_peak": 8.0 * nd / t_d / PEAK_HBM}
-    kernels["sdot_kernel (2^28)"] = dot
-    del dx, dy
+        t_d = timer.run(fn, max(10, args.steps))
+        dot = {"us": t_d * 1e6, "gbps": 8.0 * nd / t_d / 1e9,
+               "frac_of_hbm_peak": 8.0 * nd / t_d / PEAK_HBM}
+        kernels["sdot_kernel (2^28)"] = dot
+        del dx, dy
     sync()
 
     # ------------------------------------------------ grouped and serving --
-    grouped = grouped_leg(pkg, lib, torch, dev, stream, timer, args.steps)
-    sync()
-    serving = serving_leg(pkg, torch, dev, stream, args.serve_rate, args.serve_count,
-                          args.serve_slots)
-    sync()
+    grouped, serving = None, None
+    if "grouped" in legs:
+        grouped = grouped_leg(pkg, lib, torch, dev, stream, timer, args.steps)
+        sync()
+    if "serving" in legs:
+        serving = serving_leg(pkg, torch, dev, stream, args.serve_rate, args.serve_count,
+                              args.serve_slots)
+        sync()
 
     # ------------------------------------------------------- CPU baseline --
     cpu = None
-    if not args.no_cpu:
+    if "cpu" in legs:
         cpu = cpu_baseline(args.cpu_budget)
 
     res = {
@@ -245,7 +271,8 @@ def main():
         "wall_s_timed_region": round(wall, 4),
         "split_bf16_p3_step_ms": round(t_step3 * 1e3, 4),
         "plain_f32": plain,
-        "speedup_split_vs_plain": round(plain["row"]["ms"] / (t_step * 1e3), 3),
+        "speedup_split_vs_plain": (round(plain["row"]["ms"] / (t_step * 1e3), 3)
+                                   if plain else None),
         "errors": errors,
         "gemv": gemv,
         "dot": dot,

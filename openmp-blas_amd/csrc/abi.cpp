@@ -65,6 +65,27 @@ omb_status get_workspace(hipStream_t stream, size_t bytes, Workspace** out) {
     return OMB_OK;
 }
 
+int device_cu_count() {
+    static std::mutex mu;
+    static std::map<int, int> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return 256;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(dev);
+    if (it != cache.end()) return it->second;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus < 1) {
+        (void)hipGetLastError();
+        cus = 256;
+    }
+    cache[dev] = cus;
+    return cus;
+}
+
 omb_status resolve_layout(const char* name, int64_t rows, int64_t cols, int64_t rs, int64_t cs,
                           bool* unit_col, int64_t* ld) {
     if (rs < 1 || cs < 1) {

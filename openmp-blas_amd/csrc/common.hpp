@@ -43,6 +43,9 @@ struct Workspace {
     static constexpr int kTickets = 1 << 16;
 };
 
+// Compute units of the current device (cached per device).
+int device_cu_count();
+
 // Workspace of `stream` on the current device with at least `bytes` of buffer.
 omb_status get_workspace(hipStream_t stream, size_t bytes, Workspace** ws);
 
@@ -116,11 +119,24 @@ __host__ __device__ inline float rng_uniform(uint64_t key, uint64_t i) {
     return (float)(h >> 40) * 0x1.0p-23f - 1.0f;
 }
 
-// 128-bit streaming load that does not keep the line in cache.
-typedef float f32x4_nt __attribute__((ext_vector_type(4)));
-__device__ inline float4 nt_load4(const float4* p) {
-    f32x4_nt v = __builtin_nontemporal_load(reinterpret_cast<const f32x4_nt*>(p));
+// 128-bit loads of 4 consecutive floats at any 4-byte aligned address (the
+// type tells the compiler that only 4-byte alignment is known; gfx950 serves
+// it with one dwordx4 load).  The _nt form does not keep the line in cache.
+typedef float f32x4_u __attribute__((ext_vector_type(4), aligned(4)));
+__device__ inline float4 ld4u_nt(const float* p) {
+    f32x4_u v = __builtin_nontemporal_load(reinterpret_cast<const f32x4_u*>(p));
     return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ inline float4 ld4u(const float* p) {
+    f32x4_u v = *reinterpret_cast<const f32x4_u*>(p);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ inline float dot4(float4 a, float4 b, float acc) {
+    acc = fmaf(a.x, b.x, acc);
+    acc = fmaf(a.y, b.y, acc);
+    acc = fmaf(a.z, b.z, acc);
+    acc = fmaf(a.w, b.w, acc);
+    return acc;
 }
 
 // Butterfly sum over the 64 lanes of a wavefront; every lane gets the total.

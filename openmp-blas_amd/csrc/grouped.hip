@@ -50,7 +50,6 @@ struct DGemv {
     const float* x;
     float* y;
     float alpha, beta;
-    int vec;
 };
 
 enum Cls : int { C32 = 0, C64 = 1, C128x64 = 2, CSPLIT = 3, CGEMV = 4, NCLS = 5 };
@@ -181,7 +180,6 @@ __device__ __forceinline__ void tile_split(const DGemm& g, int tm, int tn, uint4
     const int frow = lane & 31, fh = lane >> 5;
     const int64_t m0 = (int64_t)tm * P, n0 = (int64_t)tn * P;
     const int64_t M = g.M, N = g.N, K = g.K;
-    const bool vec = ((reinterpret_cast<uintptr_t>(g.A) & 15) == 0) && (g.lda % 4 == 0);
     // loader roles: A row ra, k quarter qa; B^T row rb (= column of B), k quarter qb
     const int ra = tid >> 2, qa = tid & 3;
     const int rb = tid & 63, qb = tid >> 6;
@@ -195,8 +193,8 @@ __device__ __forceinline__ void tile_split(const DGemm& g, int tm, int tn, uint4
         float va[4], vb[4];
         {
             const int64_t m = m0 + ra, k = k0 + 4 * qa;
-            if (vec && m < M && k + 3 < K) {
-                const float4 t = *reinterpret_cast<const float4*>(g.A + m * g.lda + k);
+            if (m < M && k + 3 < K) {
+                const float4 t = ld4u(g.A + m * g.lda + k);
                 va[0] = t.x; va[1] = t.y; va[2] = t.z; va[3] = t.w;
             } else {
 #pragma unroll
@@ -237,16 +235,13 @@ __device__ __forceinline__ void tile_split(const DGemm& g, int tm, int tn, uint4
 
 // ------------------------------------------------------------- GEMV item --
 __device__ __forceinline__ void gemv_item(const DGemv& v, int chunk) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    // wavefront w takes rows chunk*64 + 16w .. +15, four at a time
-#pragma unroll 1
-    for (int q = 0; q < 4; ++q) {
-        const int64_t r0 = (int64_t)chunk * 64 + wave * 16 + q * kGemvRows;
-        if (r0 >= v.M) break;
-        const int nr = (int)(v.M - r0 < kGemvRows ? v.M - r0 : kGemvRows);
-        if (v.vec) gemv_rows_wave<true, 2>(r0, nr, v.N, v.alpha, v.A, v.lda, v.x, v.beta, v.y, lane);
-        else gemv_rows_wave<false, 2>(r0, nr, v.N, v.alpha, v.A, v.lda, v.x, v.beta, v.y, lane);
-    }
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // wavefront w takes rows chunk*64 + 16w .. +15 in one pass (16 rows of
+    // 128-bit loads in flight per lane)
+    const int64_t r0 = (int64_t)chunk * 64 + wave * 16;
+    if (r0 >= v.M) return;
+    const int nr = (int)(v.M - r0 < 16 ? v.M - r0 : 16);
+    gemv_rows_wave<16, 1>(r0, nr, v.N, v.alpha, v.A, v.lda, v.x, v.beta, v.y, lane);
 }
 
 // ------------------------------------------------------- persistent kernel --
@@ -457,8 +452,7 @@ omb_status build_plan(const omb_gemm_desc* gemms, int32_t n_gemm, const omb_gemv
     }
     for (int i = 0; i < n_gemv; ++i) {
         const omb_gemv_desc& d = gemvs[i];
-        v[i] = DGemv{d.M, d.N, d.A, d.lda, d.x, d.y, d.alpha, d.beta,
-                     gemv_row_vec(d.A, d.lda, d.x) ? 1 : 0};
+        v[i] = DGemv{d.M, d.N, d.A, d.lda, d.x, d.y, d.alpha, d.beta};
         for (int64_t c = 0; c < ceil_div(d.M, 64); ++c) items.push_back(Item{CGEMV, i, (int)c, 0});
         per_cls[CGEMV] += (int)ceil_div(d.M, 64);
     }

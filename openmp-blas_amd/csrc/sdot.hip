@@ -3,14 +3,15 @@
 // Counterpart of dot_prod_helper (include/dot.hpp:143-174), whose three
 // regimes (serial simd, parallel-for reduction, blocked parallel reduction)
 // all reduce FP32 lane partials into per-thread and then per-team sums.
-// Here: a grid-stride loop of 128-bit loads (4 float4 of x and of y in flight
-// per lane) into FP32 lane sums, a fixed butterfly per wavefront, the four
-// wavefront sums in order, one partial per workgroup; a second one-workgroup
-// kernel adds the partials in a fixed order -- in FP64 when n >= 2^26, in
-// FP32 below.  (A second launch instead of a last-workgroup ticket: on this
+// Here: a workgroup-interleaved grid-stride loop of 128-bit loads (4 float4
+// of x and of y in flight per lane) into FP32 lane sums, a fixed butterfly
+// per wavefront, the four wavefront sums in order, one partial per
+// workgroup; a second one-workgroup kernel adds the partials in a fixed
+// order -- in FP64 when n >= 2^26, in FP32 below.  (A second launch instead of a last-workgroup ticket: on this
 // GPU an agent-scope fence writes back and invalidates the XCD's L2, which
-// costs more than the launch.)  The grid depends on n only, so the result
-// is bitwise reproducible.
+// costs more than the launch.)  The grid depends on n only and the loads do
+// not depend on alignment, so the result is bitwise reproducible for any
+// placement of the operands.
 #include <algorithm>
 
 #include "common.hpp"
@@ -20,51 +21,36 @@ namespace {
 
 constexpr int NT = 256;
 constexpr int UNROLL = 4;
-constexpr int kMaxBlocks = 2048;
+constexpr int kMaxBlocks = 1024;
 constexpr int64_t kF64Combine = int64_t(1) << 26;
 
-__device__ inline float dot4(float4 a, float4 b, float acc) {
-    acc = fmaf(a.x, b.x, acc);
-    acc = fmaf(a.y, b.y, acc);
-    acc = fmaf(a.z, b.z, acc);
-    acc = fmaf(a.w, b.w, acc);
-    return acc;
-}
-
-template <bool VEC>
 __global__ __launch_bounds__(NT) void sdot_kernel(int64_t n, const float* __restrict__ x,
                                                   const float* __restrict__ y, float* result,
                                                   double* partial) {
     const int64_t gtid = (int64_t)blockIdx.x * NT + threadIdx.x;
     const int64_t nthr = (int64_t)gridDim.x * NT;
-    const int lane = threadIdx.x & 63;
     float acc = 0.f;
-    int64_t tail = 0;
-    if (VEC) {
-        // Each wavefront streams one contiguous segment: per step 4 KB of x
-        // and 4 KB of y (4 x 64 lanes x 16 bytes).
-        const int64_t n4 = n >> 2;
-        const float4* x4 = reinterpret_cast<const float4*>(x);
-        const float4* y4 = reinterpret_cast<const float4*>(y);
-        const int64_t waves = (int64_t)gridDim.x * (NT / 64);
-        const int64_t wave = (int64_t)blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
-        const int64_t per = round_up(ceil_div(n4, waves), 64 * UNROLL);
-        const int64_t beg = wave * per;
-        const int64_t end = beg + per < n4 ? beg + per : n4;
-        int64_t i = beg + lane;
-        for (; i + 64 * (UNROLL - 1) < end; i += 64 * UNROLL) {
-            float4 a[UNROLL], b[UNROLL];
+    // Workgroup-interleaved grid stride over 4-float groups (128-bit loads at
+    // any alignment): each step a workgroup reads 16 KB of x and of y
+    // contiguously (UNROLL groups per thread, 4 KB apart), and the grid
+    // sweeps HBM as one front (88% of peak measured at 2^28, against 75%
+    // with one contiguous segment per wavefront).
+    const int64_t n4 = n >> 2;
+    const int64_t step = (int64_t)gridDim.x * NT * UNROLL;
+    int64_t i = (int64_t)blockIdx.x * NT * UNROLL + threadIdx.x;
+    for (; i + NT * (UNROLL - 1) < n4; i += step) {
+        float4 a[UNROLL], b[UNROLL];
 #pragma unroll
-            for (int u = 0; u < UNROLL; ++u) {
-                a[u] = nt_load4(&x4[i + 64 * u]);
-                b[u] = nt_load4(&y4[i + 64 * u]);
-            }
-#pragma unroll
-            for (int u = 0; u < UNROLL; ++u) acc = dot4(a[u], b[u], acc);
+        for (int u = 0; u < UNROLL; ++u) {
+            a[u] = ld4u_nt(x + 4 * (i + NT * u));
+            b[u] = ld4u_nt(y + 4 * (i + NT * u));
         }
-        for (; i < end; i += 64) acc = dot4(x4[i], y4[i], acc);
-        tail = n4 << 2;
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) acc = dot4(a[u], b[u], acc);
     }
+    // The last, partial step of this workgroup (its next step starts past n4).
+    for (; i < n4; i += NT) acc = dot4(ld4u(x + 4 * i), ld4u(y + 4 * i), acc);
+    const int64_t tail = n4 << 2;
     for (int64_t i = tail + gtid; i < n; i += nthr) acc = fmaf(x[i], y[i], acc);
 
     // Workgroup sum: wavefront butterfly, then the 4 wave sums in order.
@@ -110,8 +96,9 @@ __global__ void zero_scalar_kernel(float* r) { *r = 0.f; }
 }  // namespace
 
 int64_t sdot_blocks(int64_t n) {
-    // ~8 float4 of each operand per thread at least; up to 8 workgroups per CU.
-    return std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, ceil_div(n, NT * 4 * 8)));
+    // One full step (NT * UNROLL float4 of each operand) per workgroup at
+    // least; 1024 workgroups (4 per CU) keep enough loads in flight.
+    return std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, ceil_div(n, NT * 4 * UNROLL)));
 }
 
 omb_status launch_sdot_ws(int64_t n, const float* x, const float* y, float* result,
@@ -122,15 +109,8 @@ omb_status launch_sdot_ws(int64_t n, const float* x, const float* y, float* resu
         return OMB_OK;
     }
     const int64_t blocks = sdot_blocks(n);
-    const bool vec = ((reinterpret_cast<uintptr_t>(x) & 15) == 0) &&
-                     ((reinterpret_cast<uintptr_t>(y) & 15) == 0);
     const int f64 = n >= kF64Combine ? 1 : 0;
-    if (vec)
-        hipLaunchKernelGGL(sdot_kernel<true>, dim3((unsigned)blocks), dim3(NT), 0, s, n, x, y,
-                           result, part);
-    else
-        hipLaunchKernelGGL(sdot_kernel<false>, dim3((unsigned)blocks), dim3(NT), 0, s, n, x, y,
-                           result, part);
+    hipLaunchKernelGGL(sdot_kernel, dim3((unsigned)blocks), dim3(NT), 0, s, n, x, y, result, part);
     if (blocks > 1)
         hipLaunchKernelGGL(sdot_combine_kernel, dim3(1), dim3(NT), 0, s, part, (int)blocks, f64,
                            result);

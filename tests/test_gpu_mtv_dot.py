@@ -166,3 +166,55 @@ def test_dot_edge_cases(dev, ompblas):
         ompblas.dot_prod(r, a, a)()
         want = sum(float(t) ** 2 for t in range(off + 1, off + 31))
         assert float(r.item()) == want
+
+
+@pytest.mark.parametrize("order", ["F", "C"])
+def test_gemv_bits_do_not_depend_on_alignment(dev, ompblas, order):
+    """The same values at 16-byte aligned and at 4/8/12-byte offset addresses,
+    with lda a multiple of 4 or not, give bitwise the same y."""
+    M, N = 1037, 2051
+    src = torch.empty(M * N, device=dev)
+    ompblas.fill_uniform(src, 9, 0)
+    x0 = torch.empty(N, device=dev)
+    ompblas.fill_uniform(x0, 9, 1)
+    rows, cols = (M, N) if order == "C" else (N, M)
+    ys = []
+    for off, pad in ((0, 0), (1, 0), (2, 3), (3, 1)):
+        ld = cols + pad
+        buf = torch.zeros(off + rows * ld, device=dev)
+        view = buf[off:].view(rows, ld)[:, :cols]
+        view.copy_(src.view(rows, cols))
+        a = view if order == "C" else view.t()          # M x N either way
+        xb = torch.zeros(off + N, device=dev)
+        xb[off:].copy_(x0)
+        x = xb[off:]
+        y = torch.zeros(M, device=dev)
+        helpers.assert_view_in_bounds(a, M, N, a.stride(0), a.stride(1))
+        ompblas.check(ompblas.sgemv(M, N, 1.0, a.data_ptr(), a.stride(0), a.stride(1),
+                                    x.data_ptr(), 0.0, y.data_ptr()))
+        ys.append(y)
+    for y in ys[1:]:
+        assert torch.equal(y, ys[0])
+    a64 = src.view(rows, cols).double()
+    want = (a64 if order == "C" else a64.t()) @ x0.double()
+    assert torch.allclose(ys[0].double(), want, rtol=0, atol=1e-3)
+
+
+def test_dot_bits_do_not_depend_on_alignment(dev, ompblas):
+    n = (1 << 20) + 7
+    x0 = torch.empty(n, device=dev)
+    y0 = torch.empty(n, device=dev)
+    ompblas.fill_uniform(x0, 9, 2)
+    ompblas.fill_uniform(y0, 9, 3)
+    rs = []
+    for ox, oy in ((0, 0), (1, 0), (0, 3), (2, 1), (3, 3)):
+        xb = torch.zeros(ox + n, device=dev)
+        yb = torch.zeros(oy + n, device=dev)
+        xb[ox:].copy_(x0)
+        yb[oy:].copy_(y0)
+        r = torch.zeros(1, device=dev)
+        ompblas.check(ompblas.sdot(n, xb[ox:].data_ptr(), yb[oy:].data_ptr(), r.data_ptr()))
+        rs.append(float(r.item()))
+    assert all(r == rs[0] for r in rs), rs
+    want = float(torch.dot(x0.double(), y0.double()))
+    assert abs(rs[0] - want) <= 1e-4 * float(torch.dot(x0.abs().double(), y0.abs().double()))
