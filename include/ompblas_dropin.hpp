@@ -1,0 +1,204 @@
+// C++ drop-in for the reference's pointer-level BLAS helpers.
+//
+//   amt::mtm_helper      include/mtm.hpp:116-122   C += A*B
+//   amt::mtv_helper      include/mtv.hpp:15-22     first_order (column-major A): c += A*b
+//                        include/mtv.hpp:72-79     last_order  (row-major A):    c  = A*b
+//   amt::dot_prod_helper include/dot.hpp:143-149   *c = sum a[i]*b[i]
+//
+// Same template signatures, argument meanings (n* = extents {rows, cols},
+// w* = strides {row stride, column stride}) and accumulate / overwrite
+// semantics as the reference; the work runs on the gfx950 kernels behind the
+// C ABI in ompblas.h.  Pointers may be host or device memory: device
+// pointers pass straight through, host buffers are staged to the device
+// (outputs are copied back).  Every call returns with its result in place,
+// like the reference's synchronous OpenMP calls.
+//
+// Deviations, by necessity: failures throw std::runtime_error carrying the
+// library's message (the reference's mtv/dot helpers are noexcept and cannot
+// fail); `max_threads` is accepted and ignored (the GPU has no thread team to
+// size); only `float` is computed -- other value types throw.
+//
+// The reference's layout tags are boost::numeric::ublas::layout::first_order
+// and last_order.  This header defines stand-ins with the same names under
+// amt::layout; code that keeps using Boost's tags adds one line per tag:
+//   template <> struct amt::layout_traits<boost::numeric::ublas::layout::first_order>
+//       : amt::layout_traits<amt::layout::first_order> {};
+#ifndef OMPBLAS_DROPIN_HPP
+#define OMPBLAS_DROPIN_HPP
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+
+#include "ompblas.h"
+
+namespace amt {
+
+namespace layout {
+struct first_order {};
+struct last_order {};
+}  // namespace layout
+
+template <class Tag>
+struct layout_traits;  // `first` is true for column-major ("first order")
+template <>
+struct layout_traits<layout::first_order> {
+    static constexpr bool first = true;
+};
+template <>
+struct layout_traits<layout::last_order> {
+    static constexpr bool first = false;
+};
+
+namespace dropin {
+
+// Process-wide choices of the drop-in layer: the HIP stream its calls use
+// (nullptr = the default stream) and the arithmetic of mtm_helper.
+struct options {
+    omb_stream_t stream = nullptr;
+    int32_t gemm_mode = OMB_MODE_PLAIN_F32;
+};
+inline options& settings() {
+    static options o;
+    return o;
+}
+
+inline void check(omb_status st, const char* what) {
+    if (st != OMB_OK) throw std::runtime_error(std::string(what) + ": " + omb_last_error());
+}
+
+// Elements spanned by a rows x cols view with strides (rs, cs).
+inline std::size_t span(int64_t rows, int64_t cols, int64_t rs, int64_t cs) {
+    if (rows <= 0 || cols <= 0) return 0;
+    return static_cast<std::size_t>((rows - 1) * rs + (cols - 1) * cs + 1);
+}
+
+// A device view of `count` elements at `p`: device memory passes through,
+// host memory is copied to a device buffer (when `in`) and back by
+// finish() (when `out`).
+template <class T>
+class staged {
+   public:
+    staged(T* p, std::size_t count, bool in, bool out, omb_stream_t s)
+        : host_(p), count_(count), out_(out), stream_(s) {
+        if (!p || count == 0 || omb_is_device_pointer(p)) {
+            dev_ = p;
+            return;
+        }
+        void* d = nullptr;
+        check(omb_device_alloc(&d, count * sizeof(T)), "amt drop-in staging");
+        dev_ = static_cast<T*>(d);
+        owned_ = true;
+        if (in) check(omb_memcpy_h2d(d, host_, count * sizeof(T), stream_), "amt drop-in staging");
+    }
+    staged(const staged&) = delete;
+    staged& operator=(const staged&) = delete;
+    ~staged() {
+        if (owned_) {
+            (void)omb_stream_synchronize(stream_);
+            (void)omb_device_free(const_cast<std::remove_const_t<T>*>(dev_));
+        }
+    }
+    T* get() const { return dev_; }
+    void finish() {
+        if (owned_ && out_)
+            check(omb_memcpy_d2h(const_cast<std::remove_const_t<T>*>(host_), dev_,
+                                 count_ * sizeof(T), stream_),
+                  "amt drop-in staging");
+        check(omb_stream_synchronize(stream_), "amt drop-in");
+    }
+
+   private:
+    T* host_;
+    T* dev_ = nullptr;
+    std::size_t count_;
+    bool out_;
+    bool owned_ = false;
+    omb_stream_t stream_;
+};
+
+}  // namespace dropin
+
+// C += A*B; M = na[0], K = na[1], N = nb[1]; any row/column-major mix
+// (include/mtm.hpp:116-206).  OutLayout only picks the reference's
+// micro-kernel store order and does not change the result.
+template <class OutLayout, class ValueType, class SizeType>
+void mtm_helper(ValueType* c, SizeType const* nc, SizeType const* wc, ValueType const* a,
+                SizeType const* na, SizeType const* wa, ValueType const* b, SizeType const* nb,
+                SizeType const* wb, OutLayout) {
+    if constexpr (!std::is_same_v<ValueType, float>) {
+        throw std::runtime_error("amt::mtm_helper: only float is implemented on the GPU");
+    } else {
+        (void)nc;
+        const int64_t M = static_cast<int64_t>(na[0]), K = static_cast<int64_t>(na[1]);
+        const int64_t N = static_cast<int64_t>(nb[1]);
+        const int64_t rsA = wa[0], csA = wa[1], rsB = wb[0], csB = wb[1], rsC = wc[0], csC = wc[1];
+        auto& o = dropin::settings();
+        dropin::staged<const float> A(a, dropin::span(M, K, rsA, csA), true, false, o.stream);
+        dropin::staged<const float> B(b, dropin::span(K, N, rsB, csB), true, false, o.stream);
+        dropin::staged<float> C(c, dropin::span(M, N, rsC, csC), true, true, o.stream);
+        dropin::check(omb_sgemm(M, N, K, 1.f, A.get(), rsA, csA, B.get(), rsB, csB, 1.f, C.get(),
+                                rsC, csC, o.gemm_mode, o.stream),
+                      "amt::mtm_helper");
+        C.finish();
+    }
+}
+
+// first_order: c += A*b with M = nc[0]*nc[1], N = nb[0]*nb[1], A strides
+// (wa[0], wa[1]) (include/mtv.hpp:15-70).  last_order: c = A*b with
+// M = na[0], N = nb[0]*nb[1], rows max(wa[0], wa[1]) apart (include/mtv.hpp:72-98).
+template <class ValueType, class SizeType, class Order>
+void mtv_helper(ValueType* c, SizeType const* nc, ValueType const* a, SizeType const* na,
+                SizeType const* wa, ValueType const* b, SizeType const* nb, int max_threads,
+                Order) {
+    (void)max_threads;
+    if constexpr (!std::is_same_v<ValueType, float>) {
+        throw std::runtime_error("amt::mtv_helper: only float is implemented on the GPU");
+    } else {
+        const bool first = layout_traits<Order>::first;
+        const int64_t N = static_cast<int64_t>(nb[0] * nb[1]);
+        int64_t M, rs, cs;
+        if (first) {
+            M = static_cast<int64_t>(nc[0] * nc[1]);
+            rs = wa[0];
+            cs = wa[1];
+        } else {
+            M = static_cast<int64_t>(na[0]);
+            rs = static_cast<int64_t>(wa[0] > wa[1] ? wa[0] : wa[1]);
+            cs = 1;
+        }
+        auto& o = dropin::settings();
+        dropin::staged<const float> A(a, dropin::span(M, N, rs, cs), true, false, o.stream);
+        dropin::staged<const float> X(b, static_cast<std::size_t>(N), true, false, o.stream);
+        dropin::staged<float> Y(c, static_cast<std::size_t>(M), first, true, o.stream);
+        dropin::check(omb_sgemv(M, N, 1.f, A.get(), rs, cs, X.get(), first ? 1.f : 0.f, Y.get(),
+                                o.stream),
+                      "amt::mtv_helper");
+        Y.finish();
+    }
+}
+
+// *c = sum_i a[i]*b[i], n = na[0]*na[1] (include/dot.hpp:143-174).
+template <class ValueType, class SizeType>
+void dot_prod_helper(ValueType* c, ValueType const* a, SizeType const* na, ValueType const* b,
+                     SizeType const* nb, int max_threads) {
+    (void)nb;
+    (void)max_threads;
+    if constexpr (!std::is_same_v<ValueType, float>) {
+        throw std::runtime_error("amt::dot_prod_helper: only float is implemented on the GPU");
+    } else {
+        const int64_t n = static_cast<int64_t>(na[0] * na[1]);
+        auto& o = dropin::settings();
+        dropin::staged<const float> X(a, static_cast<std::size_t>(n), true, false, o.stream);
+        dropin::staged<const float> Y(b, static_cast<std::size_t>(n), true, false, o.stream);
+        dropin::staged<float> R(c, 1, false, true, o.stream);
+        dropin::check(omb_sdot(n, X.get(), Y.get(), R.get(), o.stream), "amt::dot_prod_helper");
+        R.finish();
+    }
+}
+
+}  // namespace amt
+
+#endif  // OMPBLAS_DROPIN_HPP

@@ -46,37 +46,21 @@ __global__ __launch_bounds__(NT) void sgemv_row_kernel(int64_t M, int64_t N, flo
 constexpr int COL_ROWS = 256;     // rows per workgroup: 4 per lane, one float4 per column
 constexpr int COL_UNROLL = 16;    // columns in flight per wavefront (16 KB per wavefront)
 
-// Column-major A: lane l of every wavefront owns rows r0 + 4l .. r0 + 4l + 3,
-// so each column is read as 1 KB contiguous per wavefront; the 4 wavefronts
-// of a workgroup take interleaved columns of the chunk and their partial
-// sums are added in LDS in wavefront order.  The launcher sizes the grid to
-// about one workgroup per CU: fewer, longer column streams keep HBM pages
-// open (measured 88% of peak at 16384^2 against 68% with 1024 workgroups).
-__global__ __launch_bounds__(NT) void sgemv_col_kernel(int64_t M, int64_t N, float alpha,
-                                                       const float* __restrict__ A, int64_t lda,
-                                                       const float* __restrict__ x, float beta,
-                                                       float* __restrict__ y, float* partial,
-                                                       int64_t mpad, int64_t cols_per_chunk,
-                                                       int nchunks) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t r0 = (int64_t)blockIdx.x * COL_ROWS, ch = blockIdx.y;
-    const int64_t c0 = ch * cols_per_chunk;
-    const int64_t c1 = std::min<int64_t>(N, c0 + cols_per_chunk);
-    const int64_t r = r0 + 4 * lane;  // this lane's 4 rows
-    // Lanes whose 4 rows are all inside A use 128-bit loads (any alignment);
-    // the lane holding the last 1-3 rows, and lanes past M, load scalars.
-    // Either way row r+e adds its columns in the same order.
-    const bool full = r + 3 < M;
-    const float* ap = A + r;
-    // Four accumulator sets (columns j + 4u, set u % 4) keep each FP32
-    // running sum to a quarter of the chunk; the sets are added in order.
-    float acc[4][4] = {};
-    auto load = [&](int64_t j) -> float4 {
-        const float* c = ap + j * lda;
-        if (full) return ld4u_nt(c);
-        return make_float4(r < M ? c[0] : 0.f, r + 1 < M ? c[1] : 0.f, r + 2 < M ? c[2] : 0.f, 0.f);
+// One lane's sums over columns c0+wave, c0+wave+4, ... < c1 of its 4 rows at
+// ap: FULL lanes (all 4 rows inside A) use 128-bit loads at any alignment,
+// the others guarded scalars (rows past `valid` read as 0).  Four
+// accumulator sets (column j + 4u goes to set u % 4) keep each FP32 running
+// sum to a quarter of the chunk; both forms add in the same order.
+template <bool FULL>
+__device__ __forceinline__ void col_lane_sums(const float* __restrict__ ap, int64_t lda,
+                                              const float* __restrict__ x, int64_t j, int64_t c1,
+                                              int valid, float (&acc)[4][4]) {
+    auto load = [&](int64_t col) -> float4 {
+        const float* c = ap + col * lda;
+        if (FULL) return ld4u_nt(c);
+        return make_float4(valid > 0 ? c[0] : 0.f, valid > 1 ? c[1] : 0.f,
+                           valid > 2 ? c[2] : 0.f, 0.f);
     };
-    int64_t j = c0 + wave;
     for (; j + 4 * (COL_UNROLL - 1) < c1; j += 4 * COL_UNROLL) {
         float4 av[COL_UNROLL];
         float xv[COL_UNROLL];
@@ -103,6 +87,31 @@ __global__ __launch_bounds__(NT) void sgemv_col_kernel(int64_t M, int64_t N, flo
         a[2] = fmaf(av.z, xv, a[2]);
         a[3] = fmaf(av.w, xv, a[3]);
     }
+}
+
+// Column-major A: lane l of every wavefront owns rows r0 + 4l .. r0 + 4l + 3,
+// so each column is read as 1 KB contiguous per wavefront; the 4 wavefronts
+// of a workgroup take interleaved columns of the chunk and their partial
+// sums are added in LDS in wavefront order.  The launcher sizes the grid to
+// about one workgroup per CU: fewer, longer column streams keep HBM pages
+// open (measured 88% of peak at 16384^2 against 68% with 1024 workgroups).
+__global__ __launch_bounds__(NT) void sgemv_col_kernel(int64_t M, int64_t N, float alpha,
+                                                       const float* __restrict__ A, int64_t lda,
+                                                       const float* __restrict__ x, float beta,
+                                                       float* __restrict__ y, float* partial,
+                                                       int64_t mpad, int64_t cols_per_chunk,
+                                                       int nchunks) {
+    // the wavefront index is uniform: its column indices (and x) stay scalar
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t r0 = (int64_t)blockIdx.x * COL_ROWS, ch = blockIdx.y;
+    const int64_t c0 = ch * cols_per_chunk;
+    const int64_t c1 = std::min<int64_t>(N, c0 + cols_per_chunk);
+    const int64_t r = r0 + 4 * lane;  // this lane's 4 rows
+    float acc[4][4] = {};
+    if (r + 3 < M)
+        col_lane_sums<true>(A + r, lda, x, c0 + wave, c1, 4, acc);
+    else if (r < M)  // the lane holding the last 1-3 rows
+        col_lane_sums<false>(A + r, lda, x, c0 + wave, c1, (int)(M - r), acc);
     // Wavefront partials -> LDS, summed in wavefront order (thread t: row r0 + t).
     __shared__ float red[4][COL_ROWS];
 #pragma unroll
@@ -124,25 +133,19 @@ __global__ __launch_bounds__(NT) void sgemv_col_kernel(int64_t M, int64_t N, flo
     partial[ch * mpad + row] = sum;
 }
 
-// y[r] = alpha * sum_c partial[c][r] + beta * y[r], chunks added in order.
+// y[r] = alpha * sum_c partial[c][r] + beta * y[r], chunks added in order
+// (one row per thread).
 __global__ __launch_bounds__(NT) void sgemv_col_combine_kernel(int64_t M, float alpha,
                                                                const float* __restrict__ partial,
                                                                int64_t mpad, int nchunks,
                                                                float beta, float* __restrict__ y) {
-    const int64_t r = ((int64_t)blockIdx.x * NT + threadIdx.x) * 4;
+    const int64_t r = (int64_t)blockIdx.x * NT + threadIdx.x;
     if (r >= M) return;
-    const int nv = (int)std::min<int64_t>(4, M - r);
-    float t[4] = {0.f, 0.f, 0.f, 0.f};
-    for (int c = 0; c < nchunks; ++c) {
-        const float* pp = partial + (int64_t)c * mpad + r;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) t[e] += pp[e];
-    }
-    for (int e = 0; e < nv; ++e) {
-        float out = alpha * t[e];
-        if (beta != 0.f) out += beta * y[r + e];
-        y[r + e] = out;
-    }
+    float t = 0.f;
+    for (int c = 0; c < nchunks; ++c) t += partial[(int64_t)c * mpad + r];
+    float out = alpha * t;
+    if (beta != 0.f) out += beta * y[r];
+    y[r] = out;
 }
 
 }  // namespace
@@ -195,7 +198,7 @@ omb_status launch_sgemv_col(int64_t M, int64_t N, float alpha, const float* A, i
     hipLaunchKernelGGL(sgemv_col_kernel, grid, dim3(NT), 0, s, M, N, alpha, A, lda, x, beta, y,
                        part, mpad, cpc, (int)nchunks);
     if (nchunks > 1)
-        hipLaunchKernelGGL(sgemv_col_combine_kernel, dim3((unsigned)ceil_div(mpad / 4, NT)),
+        hipLaunchKernelGGL(sgemv_col_combine_kernel, dim3((unsigned)ceil_div(M, NT)),
                            dim3(NT), 0, s, M, alpha, part, mpad, (int)nchunks, beta, y);
     OMB_HIP_TRY(hipGetLastError());
     return OMB_OK;

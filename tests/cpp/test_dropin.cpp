@@ -1,0 +1,288 @@
+// Tests of the C++ drop-in layer (ompblas_dropin.hpp) in the shape of the
+// reference's Catch2 suites: test/test.mtm.cpp (8 layout combinations,
+// sizes 2..31), test/test.mtv.cpp (first / last order, sizes 2..511 and
+// 32 Ki), test/test.dot_prod.cpp (1..s closed form).  Inputs are
+// rand() % 100 as in test/test_utils.hpp; the expected values come from
+// integer brute force (exact while sums stay below 2^24) instead of BLIS,
+// and from a relative bound for the 32 Ki case (the reference's Approx).
+// Host buffers exercise the staging path, device buffers the pass-through.
+//
+// Usage: test_dropin [--quick]   (exit status 0 = all passed)
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ompblas_dropin.hpp"
+
+namespace {
+
+int g_checks = 0, g_failures = 0;
+
+#define CHECK(cond, ...)                                   \
+    do {                                                   \
+        ++g_checks;                                        \
+        if (!(cond)) {                                     \
+            ++g_failures;                                  \
+            if (g_failures <= 20) {                        \
+                std::printf("FAIL %s:%d: ", __FILE__, __LINE__); \
+                std::printf(__VA_ARGS__);                  \
+                std::printf("\n");                         \
+            }                                              \
+        }                                                  \
+    } while (0)
+
+// A matrix in one storage order with uBLAS-style extents and strides.
+struct Mat {
+    std::size_t n[2];
+    std::size_t w[2];
+    std::vector<float> v;
+    Mat(std::size_t rows, std::size_t cols, bool first) : v(rows * cols) {
+        n[0] = rows;
+        n[1] = cols;
+        w[0] = first ? 1 : cols;
+        w[1] = first ? rows : 1;
+    }
+    float& at(std::size_t i, std::size_t j) { return v[i * w[0] + j * w[1]]; }
+};
+
+void rand_fill(std::vector<float>& v) {
+    for (auto& x : v) x = static_cast<float>(std::rand() % 100);
+}
+
+// Device copy of a host vector (freed by the destructor).
+struct Dev {
+    float* p = nullptr;
+    std::size_t n = 0;
+    explicit Dev(const std::vector<float>& h) : n(h.size()) {
+        void* d = nullptr;
+        amt::dropin::check(omb_device_alloc(&d, n * sizeof(float)), "alloc");
+        p = static_cast<float*>(d);
+        amt::dropin::check(omb_memcpy_h2d(p, h.data(), n * sizeof(float), nullptr), "h2d");
+        amt::dropin::check(omb_stream_synchronize(nullptr), "sync");
+    }
+    std::vector<float> get() const {
+        std::vector<float> h(n);
+        amt::dropin::check(omb_memcpy_d2h(h.data(), p, n * sizeof(float), nullptr), "d2h");
+        amt::dropin::check(omb_stream_synchronize(nullptr), "sync");
+        return h;
+    }
+    ~Dev() { (void)omb_device_free(p); }
+};
+
+template <class Tag>
+void mtm_case(bool fc, bool fa, bool fb, std::size_t sz, bool device, const char* name) {
+    Mat A(sz, sz, fa), B(sz, sz, fb), C(sz, sz, fc);
+    rand_fill(A.v);
+    rand_fill(B.v);
+    rand_fill(C.v);  // C += A*B: a nonzero start checks the accumulation
+    std::vector<int64_t> want(sz * sz);
+    for (std::size_t i = 0; i < sz; ++i)
+        for (std::size_t j = 0; j < sz; ++j) {
+            int64_t s = static_cast<int64_t>(C.at(i, j));
+            for (std::size_t k = 0; k < sz; ++k)
+                s += static_cast<int64_t>(A.at(i, k)) * static_cast<int64_t>(B.at(k, j));
+            want[i * sz + j] = s;
+        }
+    if (device) {
+        Dev a(A.v), b(B.v), c(C.v);
+        amt::mtm_helper(c.p, C.n, C.w, static_cast<const float*>(a.p), A.n, A.w,
+                        static_cast<const float*>(b.p), B.n, B.w, Tag{});
+        C.v = c.get();
+    } else {
+        amt::mtm_helper(C.v.data(), C.n, C.w, static_cast<const float*>(A.v.data()), A.n, A.w,
+                        static_cast<const float*>(B.v.data()), B.n, B.w, Tag{});
+    }
+    for (std::size_t i = 0; i < sz; ++i)
+        for (std::size_t j = 0; j < sz; ++j)
+            CHECK(static_cast<int64_t>(C.at(i, j)) == want[i * sz + j],
+                  "mtm %s sz=%zu %s (%zu,%zu): got %.1f want %lld", name, sz,
+                  device ? "device" : "host", i, j, C.at(i, j), (long long)want[i * sz + j]);
+}
+
+void test_mtm(bool quick) {
+    const char* names[8] = {"FFF", "FFL", "FLF", "FLL", "LFF", "LFL", "LLF", "LLL"};
+    for (int l = 0; l < 8; ++l) {
+        const bool fc = !(l & 4), fa = !(l & 2), fb = !(l & 1);
+        for (std::size_t sz = 2; sz < 32; sz += quick ? 7 : 1) {
+            if (fc)
+                mtm_case<amt::layout::first_order>(fc, fa, fb, sz, false, names[l]);
+            else
+                mtm_case<amt::layout::last_order>(fc, fa, fb, sz, false, names[l]);
+        }
+        for (std::size_t sz : {2, 17, 31})
+            mtm_case<amt::layout::last_order>(fc, fa, fb, sz, true, names[l]);
+    }
+}
+
+// first order: c += A v (A column-major); last order: c = A v (row-major).
+void mtv_case(bool first, std::size_t sz, bool device) {
+    Mat A(sz, sz, first);
+    rand_fill(A.v);
+    std::vector<float> v(sz), c(sz);
+    rand_fill(v);
+    rand_fill(c);
+    std::vector<int64_t> want(sz);
+    for (std::size_t i = 0; i < sz; ++i) {
+        int64_t s = first ? static_cast<int64_t>(c[i]) : 0;
+        for (std::size_t k = 0; k < sz; ++k)
+            s += static_cast<int64_t>(A.at(i, k)) * static_cast<int64_t>(v[k]);
+        want[i] = s;
+    }
+    const std::size_t nv[2] = {1, sz}, nc[2] = {1, sz};
+    auto call = [&](float* cp, const float* ap, const float* vp) {
+        if (first)
+            amt::mtv_helper(cp, nc, ap, A.n, A.w, vp, nv, 8, amt::layout::first_order{});
+        else
+            amt::mtv_helper(cp, nc, ap, A.n, A.w, vp, nv, 8, amt::layout::last_order{});
+    };
+    if (device) {
+        Dev a(A.v), dv(v), dc(c);
+        call(dc.p, a.p, dv.p);
+        c = dc.get();
+    } else {
+        call(c.data(), A.v.data(), v.data());
+    }
+    for (std::size_t i = 0; i < sz; ++i)
+        CHECK(static_cast<int64_t>(c[i]) == want[i], "mtv %s sz=%zu %s row %zu: got %.1f want %lld",
+              first ? "first" : "last", sz, device ? "device" : "host", i, c[i],
+              (long long)want[i]);
+}
+
+void test_mtv(bool quick) {
+    for (int first = 0; first < 2; ++first) {
+        for (std::size_t sz = 2; sz < 512; sz += quick ? 37 : 1) mtv_case(first, sz, false);
+        for (std::size_t sz : {2, 255, 511}) mtv_case(first, sz, true);
+    }
+}
+
+// test/test.mtv.cpp:128-211: 32 Ki x 32 Ki (4 GiB), compared with a relative
+// bound like Catch2's Approx (the sums exceed 2^24).
+void test_mtv_large(bool first) {
+    const std::size_t sz = 32 * 1024;
+    Mat A(sz, sz, first);
+    rand_fill(A.v);
+    std::vector<float> v(sz), c(sz, 0.f);
+    rand_fill(v);
+    std::vector<double> want(sz, 0.0);
+    if (first) {  // walk A in storage order
+        for (std::size_t k = 0; k < sz; ++k)
+            for (std::size_t i = 0; i < sz; ++i) want[i] += static_cast<double>(A.v[i + k * sz]) * v[k];
+    } else {
+        for (std::size_t i = 0; i < sz; ++i)
+            for (std::size_t k = 0; k < sz; ++k) want[i] += static_cast<double>(A.v[i * sz + k]) * v[k];
+    }
+    Dev a(A.v), dv(v), dc(c);
+    A.v.clear();
+    A.v.shrink_to_fit();
+    const std::size_t nv[2] = {1, sz}, nc[2] = {1, sz};
+    if (first)
+        amt::mtv_helper(dc.p, nc, static_cast<const float*>(a.p), A.n, A.w,
+                        static_cast<const float*>(dv.p), nv, 8, amt::layout::first_order{});
+    else
+        amt::mtv_helper(dc.p, nc, static_cast<const float*>(a.p), A.n, A.w,
+                        static_cast<const float*>(dv.p), nv, 8, amt::layout::last_order{});
+    c = dc.get();
+    const double eps = 100.0 * 1.1920929e-07;  // Catch2 Approx default
+    for (std::size_t i = 0; i < sz; ++i)
+        CHECK(std::fabs(c[i] - want[i]) <= eps * (1.0 + std::fabs(want[i])),
+              "mtv large %s row %zu: got %.1f want %.1f", first ? "first" : "last", i, c[i], want[i]);
+}
+
+// test/test.dot_prod.cpp:10-28: a = 1..s, a.a = s(s+1)(2s+1)/6.
+void test_dot(bool quick) {
+    const std::size_t smax = (1u << 17) + 1;
+    std::vector<float> full(smax);
+    for (std::size_t i = 0; i < smax; ++i) full[i] = static_cast<float>(i + 1);
+    Dev dfull(full);
+    float* r_dev = nullptr;
+    {
+        void* p = nullptr;
+        amt::dropin::check(omb_device_alloc(&p, sizeof(float)), "alloc");
+        r_dev = static_cast<float*>(p);
+    }
+    const double u = std::ldexp(1.0, -24);
+    const std::size_t step = quick ? 4099 : 97;
+    for (std::size_t s = 2; s <= smax; s += (s < 300 ? 1 : step)) {
+        const std::size_t n[2] = {1, s};
+        float r = 0.f;
+        const bool device = (s % 3 == 0);
+        if (device) {
+            amt::dot_prod_helper(r_dev, static_cast<const float*>(dfull.p), n,
+                                 static_cast<const float*>(dfull.p), n, 8);
+            amt::dropin::check(omb_memcpy_d2h(&r, r_dev, sizeof(float), nullptr), "d2h");
+            amt::dropin::check(omb_stream_synchronize(nullptr), "sync");
+        } else {
+            amt::dot_prod_helper(&r, static_cast<const float*>(full.data()), n,
+                                 static_cast<const float*>(full.data()), n, 8);
+        }
+        const double exact = static_cast<double>(s) * (s + 1) * (2 * s + 1) / 6.0;
+        if (exact < 16777216.0)
+            CHECK(r == exact, "dot s=%zu: got %.1f want %.1f", s, r, exact);
+        else
+            CHECK(std::fabs(r - exact) <= (s * u / (1 - s * u)) * exact, "dot s=%zu: got %.6e want %.6e",
+                  s, r, exact);
+    }
+    (void)omb_device_free(r_dev);
+}
+
+void test_errors() {
+    // only float is computed
+    double c = 0, a[4] = {1, 2, 3, 4};
+    const std::size_t n[2] = {2, 2}, w[2] = {2, 1};
+    bool threw = false;
+    try {
+        amt::mtm_helper(&c, n, w, static_cast<const double*>(a), n, w, static_cast<const double*>(a),
+                        n, w, amt::layout::last_order{});
+    } catch (const std::runtime_error& e) {
+        threw = std::strstr(e.what(), "only float") != nullptr;
+    }
+    CHECK(threw, "double mtm_helper must throw");
+    // strides where neither is 1 are refused with the library's message
+    std::vector<float> A(64, 1.f), B(64, 1.f), C(64, 0.f);
+    const std::size_t na[2] = {3, 3}, wbad[2] = {6, 2}, wok[2] = {3, 1};
+    threw = false;
+    try {
+        amt::mtm_helper(C.data(), na, wok, static_cast<const float*>(A.data()), na, wbad,
+                        static_cast<const float*>(B.data()), na, wok, amt::layout::last_order{});
+    } catch (const std::runtime_error& e) {
+        threw = std::strstr(e.what(), "amt::mtm_helper") != nullptr;
+    }
+    CHECK(threw, "general strides must throw from mtm_helper");
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    const bool quick = argc > 1 && std::strcmp(argv[1], "--quick") == 0;
+    std::srand(0);
+    try {
+        amt::dropin::check(omb_init(), "omb_init");
+        test_mtm(quick);
+        std::printf("mtm: %d checks, %d failures\n", g_checks, g_failures);
+        test_mtv(quick);
+        std::printf("mtv: %d checks, %d failures\n", g_checks, g_failures);
+        test_dot(quick);
+        std::printf("dot: %d checks, %d failures\n", g_checks, g_failures);
+        if (!quick) {
+            test_mtv_large(true);
+            test_mtv_large(false);
+            std::printf("mtv 32Ki: %d checks, %d failures\n", g_checks, g_failures);
+        }
+        test_errors();
+        amt::dropin::settings().gemm_mode = OMB_MODE_SPLIT_BF16;
+        for (std::size_t sz : {5, 31}) mtm_case<amt::layout::last_order>(false, true, false, sz, false, "LFL split");
+        amt::dropin::settings().gemm_mode = OMB_MODE_PLAIN_F32;
+    } catch (const std::exception& e) {
+        std::printf("FAIL: exception: %s\n", e.what());
+        return 2;
+    }
+    std::printf("%s: %d checks, %d failures\n", g_failures ? "FAILED" : "PASSED", g_checks,
+                g_failures);
+    std::fflush(stdout);
+    return g_failures ? 1 : 0;
+}

@@ -163,3 +163,14 @@ def test_serving_trace_mix_and_ranges(ompblas):
     assert v.min() >= 256 and v.max() <= 16384
     d = np.array([x[2] for x in t if x[0] == 2])
     assert d.min() >= 1024 and d.max() <= 1 << 24
+
+
+def test_cpp_dropin_program_is_built_against_the_library(ompblas):
+    """build() compiles the C++ drop-in header and its test program with g++,
+    linked to the product library (not to the oracle)."""
+    import os
+    import subprocess
+    path = os.path.join(os.path.dirname(ompblas.LIB_PATH), "test_dropin")
+    assert os.path.exists(path)
+    deps = subprocess.run(["ldd", path], capture_output=True, text=True).stdout
+    assert "libompblas.so" in deps and "oracle" not in deps
