@@ -35,6 +35,22 @@ PEAK_MFMA_BF16 = 2.5e15
 PEAK_FP32 = 157.3e12
 
 
+def profiled_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` on `workload` as measured by the
+    profiler's FETCH_SIZE / WRITE_SIZE passes (scaled by a copy of known
+    size), read from the committed summary; None when it is absent."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_pmc.md")
+    try:
+        for line in open(path):
+            parts = line.split()
+            if (len(parts) == 4 and parts[0] == "traffic_bytes_per_launch"
+                    and parts[1] == kernel and parts[2] == workload + ":"):
+                return int(parts[3])
+    except OSError:
+        pass
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -265,7 +281,10 @@ def main():
         "roofline": {"bound": "mfma", "kernel": "sgemm_bf16x3_kernel<6>",
                      "achieved": round(6 * flop / t_k6 / 1e12, 2), "peak": 2500.0,
                      "unit": "TFLOP/s", "frac": round(6 * flop / t_k6 / PEAK_MFMA_BF16, 4),
-                     "traffic": None,
+                     "traffic": profiled_traffic("sgemm_bf16x3_kernel<6>", "8192^3"),
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                                     "scaled by a 256 MiB copy)",
+                     "algorithmic_bytes": 2 * 3 * n * n * 2 + 4 * n * n,
                      "work_per_launch": "6 products x 2*8192^3 = 6.597e12 MFMA flop"},
         "cpu_baseline": cpu,
         "wall_s_timed_region": round(wall, 4),
