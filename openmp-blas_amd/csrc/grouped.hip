@@ -5,9 +5,9 @@
 // Work list: every GEMM is cut into tiles of the class the autotuner chose
 // for its (log2 M, log2 N, log2 K) bucket, every GEMV into 64-row chunks;
 // items are sorted by estimated cost (longest first).  A fixed grid of
-// workgroups (CUs x occupancy) pulls items from an atomic counter; the last
-// workgroup to finish resets the counters (atomics only: no cache-flushing
-// fences, no memset node in the graph).
+// workgroups (CUs x occupancy) pulls chunks of consecutive items from an
+// atomic counter; the last workgroup to finish resets the counters (atomics
+// only: no cache-flushing fences, no memset node in the graph).
 //
 // Every tile class keeps the per-element arithmetic of the standalone kernels
 // (plain: sequential k FMAs folded every 128 k; split-BF16: the same MFMA
@@ -67,22 +67,28 @@ struct PArgs {
     int nitems;
     const DGemm* gemms;
     const DGemv* gemvs;
-    unsigned* counters;  // [0] next item, [1] workgroups done
+    unsigned* counters;  // [0] next dynamic chunk, [1] workgroups done
     int nprod;
+    int chunk;    // consecutive items per chunk
+    int nchunks;  // ceil(nitems / chunk)
 };
 
 constexpr int NT = 256;
-constexpr int BK = 16;
-constexpr int kFold = 8;  // 8 slices of 16 = 128 k per inner block (as sgemm_f32)
+constexpr int kFoldK = 128;  // k per inner accumulation block (as sgemm_f32)
 
 // ------------------------------------------------------------ plain tile --
 // BM x BN tile of C = alpha*A*B + beta*C (all row-major); 256 threads on a
 // (BM/TMR) x (BN/TNR) grid, thread (ty, tx) owns rows ty + i*H, columns
-// tx + j*W.  One LDS slice, the next one prefetched into registers.
-template <int BM, int BN, int TMR, int TNR>
+// tx + j*W.  One LDS slice of BK k, the next one prefetched into registers.
+// Small tiles take deeper slices (BK up to 64) so that enough loads are in
+// flight per step; every class sums k in order and folds the running sum
+// every 128 k, as sgemm_f32_kernel does, so the results are the same bits.
+template <int BM, int BN, int TMR, int TNR, int BK>
 __device__ __forceinline__ void tile_f32(const DGemm& g, int tm, int tn, float* smem) {
     constexpr int H = BM / TMR, W = BN / TNR;
     static_assert(H * W == NT, "thread grid must cover the workgroup");
+    static_assert(kFoldK % BK == 0, "fold boundaries on slice boundaries");
+    constexpr int kFold = kFoldK / BK;
     constexpr int LA = BM * BK / NT, LB = BN * BK / NT;
     constexpr int PA = BM + 1, PB = BN + 1;  // padded LDS rows (conflict-free stores)
     float* As = smem;            // [BK][PA]
@@ -129,7 +135,7 @@ __device__ __forceinline__ void tile_f32(const DGemm& g, int tm, int tn, float* 
         }
         __syncthreads();
         if (kt + 1 < nk) load((kt + 1) * BK);
-#pragma unroll
+#pragma unroll 16
         for (int k = 0; k < BK; ++k) {
             float a[TMR], b[TNR];
 #pragma unroll
@@ -245,40 +251,51 @@ __device__ __forceinline__ void gemv_item(const DGemv& v, int chunk) {
 }
 
 // ------------------------------------------------------- persistent kernel --
-constexpr int kSmemFloats = BK * (128 + 1) + BK * (64 + 1) + 16;  // 128x64 tile; split uses 12 KB
+// LDS: the largest plain class footprint (32x32 with BK 64, 64x64 with BK
+// 32, 128x64 with BK 16); the split tile uses 12 KB.
+constexpr int kSmemFloats = 64 * (32 + 1) * 2 + 16;
 
 template <bool BIG>
 __global__ __launch_bounds__(NT, 2) void grouped_kernel(PArgs p) {
     __shared__ __attribute__((aligned(16))) float smem[kSmemFloats];
-    __shared__ int s_item;
+    __shared__ int s_chunk;
+    // Chunks of consecutive (cost-sorted) items: workgroup b starts with
+    // chunk b, then takes chunks gridDim.x, gridDim.x + 1, ... from one
+    // device-scope counter.  Such atomics on one address serialize (about
+    // 70 ns each measured here), so one ticket per chunk instead of one per
+    // item keeps the counter off the critical path.
+    int c = blockIdx.x;
     for (;;) {
-        __syncthreads();  // the previous item is done with LDS and s_item
-        if (threadIdx.x == 0) s_item = (int)atomicAdd(&p.counters[0], 1u);
-        __syncthreads();
-        // s_item is the same for the whole workgroup: make that visible to the
-        // compiler so the descriptors live in scalar registers.
-        const int it = __builtin_amdgcn_readfirstlane(s_item);
-        if (it >= p.nitems) break;
-        Item w = p.items[it];
-        w.cls = __builtin_amdgcn_readfirstlane(w.cls);
-        w.inst = __builtin_amdgcn_readfirstlane(w.inst);
-        w.tm = __builtin_amdgcn_readfirstlane(w.tm);
-        w.tn = __builtin_amdgcn_readfirstlane(w.tn);
-        switch (w.cls) {
-            case C32: tile_f32<32, 32, 2, 2>(p.gemms[w.inst], w.tm, w.tn, smem); break;
-            case C64: tile_f32<64, 64, 4, 4>(p.gemms[w.inst], w.tm, w.tn, smem); break;
-            case C128x64:
-                if (BIG) tile_f32<128, 64, 8, 4>(p.gemms[w.inst], w.tm, w.tn, smem);
-                break;
-            case CSPLIT:
-                if (p.nprod == 3)
-                    tile_split<3>(p.gemms[w.inst], w.tm, w.tn, reinterpret_cast<uint4*>(smem));
-                else
-                    tile_split<6>(p.gemms[w.inst], w.tm, w.tn, reinterpret_cast<uint4*>(smem));
-                break;
-            case CGEMV: gemv_item(p.gemvs[w.inst], w.tm); break;
-            default: break;
+        if (c >= p.nchunks) break;
+        const int beg = c * p.chunk, end = min(beg + p.chunk, p.nitems);
+        for (int it = beg; it < end; ++it) {
+            __syncthreads();  // the previous item is done with LDS
+            Item w = p.items[it];
+            w.cls = __builtin_amdgcn_readfirstlane(w.cls);
+            w.inst = __builtin_amdgcn_readfirstlane(w.inst);
+            w.tm = __builtin_amdgcn_readfirstlane(w.tm);
+            w.tn = __builtin_amdgcn_readfirstlane(w.tn);
+            switch (w.cls) {
+                case C32: tile_f32<32, 32, 2, 2, 16>(p.gemms[w.inst], w.tm, w.tn, smem); break;
+                case C64: tile_f32<64, 64, 4, 4, 16>(p.gemms[w.inst], w.tm, w.tn, smem); break;
+                case C128x64:
+                    if (BIG) tile_f32<128, 64, 8, 4, 16>(p.gemms[w.inst], w.tm, w.tn, smem);
+                    break;
+                case CSPLIT:
+                    if (p.nprod == 3)
+                        tile_split<3>(p.gemms[w.inst], w.tm, w.tn, reinterpret_cast<uint4*>(smem));
+                    else
+                        tile_split<6>(p.gemms[w.inst], w.tm, w.tn, reinterpret_cast<uint4*>(smem));
+                    break;
+                case CGEMV: gemv_item(p.gemvs[w.inst], w.tm); break;
+                default: break;
+            }
         }
+        __syncthreads();  // every thread has read s_chunk for this chunk
+        if (threadIdx.x == 0) s_chunk = (int)gridDim.x + (int)atomicAdd(&p.counters[0], 1u);
+        __syncthreads();
+        // uniform across the workgroup: keep it (and the descriptors) scalar
+        c = __builtin_amdgcn_readfirstlane(s_chunk);
     }
     if (threadIdx.x == 0 && atomicAdd(&p.counters[1], 1u) == gridDim.x - 1) {
         atomicExch(&p.counters[0], 0u);
@@ -336,6 +353,7 @@ struct omb_group_plan {
     bool big = false;
     int grid = 0;
     int nitems = 0;
+    int chunk = 1;
     omb::DGemm* d_gemms = nullptr;
     omb::DGemv* d_gemvs = nullptr;
     omb::Item* d_items = nullptr;
@@ -404,7 +422,8 @@ void free_plan(omb_group_plan* p) {
 omb_status launch_plan(omb_group_plan* p, hipStream_t s) {
     if (p->nitems == 0) return OMB_OK;
     PArgs a{p->d_items, p->nitems, p->d_gemms, p->d_gemvs, p->d_counters,
-            p->mode == OMB_MODE_SPLIT_BF16_P3 ? 3 : 6};
+            p->mode == OMB_MODE_SPLIT_BF16_P3 ? 3 : 6, p->chunk,
+            (int)ceil_div(p->nitems, p->chunk)};
     if (p->big)
         hipLaunchKernelGGL(grouped_kernel<true>, dim3(p->grid), dim3(NT), 0, s, a);
     else
@@ -491,6 +510,10 @@ omb_status build_plan(const omb_gemm_desc* gemms, int32_t n_gemm, const omb_gemv
     if (e != hipSuccess || per_cu < 1) per_cu = 1;
     p->dev = dev;
     p->grid = std::max(1, std::min(p->nitems, cus * per_cu));
+    // up to 4 items per chunk (about 8 chunks per workgroup at most): 4x
+    // fewer tickets; longer chunks of the cost-sorted list measured slower
+    // on mixed GEMM + GEMV plans
+    p->chunk = std::max(1, std::min(4, p->nitems / (8 * p->grid)));
 
     if (capture && p->nitems > 0) {
         hipStream_t cs = nullptr;
@@ -512,7 +535,8 @@ omb_status build_plan(const omb_gemm_desc* gemms, int32_t n_gemm, const omb_gemv
         }
     }
     std::ostringstream os;
-    os << "items=" << p->nitems << " grid=" << p->grid << " kernel=" << (p->big ? "big" : "lite")
+    os << "items=" << p->nitems << " grid=" << p->grid << " chunk=" << p->chunk
+       << " kernel=" << (p->big ? "big" : "lite")
        << " mode=" << mode;
     for (int c = 0; c < NCLS; ++c)
         if (per_cls[c]) os << " " << kClsName[c] << "=" << per_cls[c];

@@ -145,12 +145,21 @@ __global__ __launch_bounds__(NT, 2) void sgemm_f32_kernel(Args p) {
         const int cur = kt & 1;
         const bool more = kt + 1 < nk;
         if (more) load_slice<A_KC, B_NC, VEC>(p, m0, n0, (kt + 1) * BK, tid, ra, rb);
+        // LDS fragments in two register sets: row k+1 is read while row k's
+        // 64 FMAs run (one set only left the LDS latency exposed every k).
+        float4 fa[2][2], fb[2][2];
+        auto fragment = [&](int k, int set) {
+            fa[set][0] = *reinterpret_cast<const float4*>(&As[cur][k][ty * 4]);
+            fa[set][1] = *reinterpret_cast<const float4*>(&As[cur][k][64 + ty * 4]);
+            fb[set][0] = *reinterpret_cast<const float4*>(&Bs[cur][k][tx * 4]);
+            fb[set][1] = *reinterpret_cast<const float4*>(&Bs[cur][k][64 + tx * 4]);
+        };
+        fragment(0, 0);
 #pragma unroll
         for (int k = 0; k < BK; ++k) {
-            float4 a0 = *reinterpret_cast<const float4*>(&As[cur][k][ty * 4]);
-            float4 a1 = *reinterpret_cast<const float4*>(&As[cur][k][64 + ty * 4]);
-            float4 b0 = *reinterpret_cast<const float4*>(&Bs[cur][k][tx * 4]);
-            float4 b1 = *reinterpret_cast<const float4*>(&Bs[cur][k][64 + tx * 4]);
+            if (k + 1 < BK) fragment(k + 1, (k + 1) & 1);
+            const float4 a0 = fa[k & 1][0], a1 = fa[k & 1][1];
+            const float4 b0 = fb[k & 1][0], b1 = fb[k & 1][1];
             const float a[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
             const float b[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
 #pragma unroll
