@@ -77,39 +77,99 @@ constexpr int NT = 256;
 constexpr int kFoldK = 128;  // k per inner accumulation block (as sgemm_f32)
 
 // ------------------------------------------------------------ plain tile --
-// BM x BN tile of C = alpha*A*B + beta*C (all row-major); 256 threads on a
-// (BM/TMR) x (BN/TNR) grid, thread (ty, tx) owns rows ty + i*H, columns
-// tx + j*W.  One LDS slice of BK k, the next one prefetched into registers.
-// Small tiles take deeper slices (BK up to 64) so that enough loads are in
-// flight per step; every class sums k in order and folds the running sum
-// every 128 k, as sgemm_f32_kernel does, so the results are the same bits.
-template <int BM, int BN, int TMR, int TNR, int BK>
+// BM x BN tile of C = alpha*A*B + beta*C (all row-major); 256 threads on an
+// H x W = (BM/TMR) x (BN/TNR) grid.  Thread (ty, tx) owns GR fragments of FR
+// consecutive rows, at rows g*(BM/GR) + ty*FR + i, and GC fragments of FC
+// consecutive columns (FR = min(TMR, 4), GR = TMR / FR; likewise for
+// columns), so every LDS read is one 64- or 128-bit access.  Slices of 16 k
+// are loaded with 128-bit global loads along the unit-stride index, staged
+// through LDS, and the next slice is prefetched into registers; LDS
+// fragments for k+1 are read while k's FMAs run.  Every element sums k in
+// order and folds the running sum every 128 k, as sgemm_f32_kernel does, so
+// the results are the same bits.
+template <int N>
+struct FragT;
+template <>
+struct FragT<2> {
+    typedef float2 type;
+};
+template <>
+struct FragT<4> {
+    typedef float4 type;
+};
+
+__device__ __forceinline__ float4 load4_guard(const float* p, int n) {
+    if (n == 4) return ld4u(p);
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (n > 0) v.x = p[0];
+    if (n > 1) v.y = p[1];
+    if (n > 2) v.z = p[2];
+    return v;
+}
+
+__device__ __forceinline__ int clamp4i(int64_t r) { return r <= 0 ? 0 : (r >= 4 ? 4 : (int)r); }
+
+template <int BM, int BN, int TMR, int TNR>
 __device__ __forceinline__ void tile_f32(const DGemm& g, int tm, int tn, float* smem) {
+    constexpr int BK = 16;
     constexpr int H = BM / TMR, W = BN / TNR;
     static_assert(H * W == NT, "thread grid must cover the workgroup");
-    static_assert(kFoldK % BK == 0, "fold boundaries on slice boundaries");
+    constexpr int FR = TMR < 4 ? TMR : 4, GR = TMR / FR;
+    constexpr int FC = TNR < 4 ? TNR : 4, GC = TNR / FC;
     constexpr int kFold = kFoldK / BK;
-    constexpr int LA = BM * BK / NT, LB = BN * BK / NT;
-    constexpr int PA = BM + 1, PB = BN + 1;  // padded LDS rows (conflict-free stores)
+    constexpr int PA = BM + 4, PB = BN + 4;  // padded LDS rows, 16-byte aligned
+    constexpr int NA4 = BM * BK / 4, NB4 = BN * BK / 4;  // float4 per slice
+    constexpr int LA = (NA4 + NT - 1) / NT, LB = (NB4 + NT - 1) / NT;
+    typedef typename FragT<FR>::type FragA;
+    typedef typename FragT<FC>::type FragB;
     float* As = smem;            // [BK][PA]
     float* Bs = smem + BK * PA;  // [BK][PB]
     const int tid = threadIdx.x, tx = tid % W, ty = tid / W;
     const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
     const int64_t M = g.M, N = g.N, K = g.K;
 
-    float ra[LA], rb[LB];
+    float4 ra[LA], rb[LB];
     auto load = [&](int64_t k0) {
 #pragma unroll
         for (int i = 0; i < LA; ++i) {
-            const int e = tid + i * NT, r = e / BK, k = e % BK;
-            const int64_t m = m0 + r, kk = k0 + k;
-            ra[i] = (m < M && kk < K) ? g.A[m * g.lda + kk] : 0.f;
+            const int e = tid + i * NT;
+            if (e < NA4) {  // A[m][k]: float4 along k
+                const int row = e >> 2, kq = (e & 3) * 4;
+                const int64_t m = m0 + row, k = k0 + kq;
+                const int n = m < M ? clamp4i(K - k) : 0;
+                ra[i] = load4_guard(g.A + (n ? m * g.lda + k : 0), n);
+            }
         }
 #pragma unroll
         for (int i = 0; i < LB; ++i) {
-            const int e = tid + i * NT, k = e / BN, c = e % BN;
-            const int64_t n = n0 + c, kk = k0 + k;
-            rb[i] = (kk < K && n < N) ? g.B[kk * g.ldb + n] : 0.f;
+            const int e = tid + i * NT;
+            if (e < NB4) {  // B[k][n]: float4 along n
+                const int kk = e / (BN / 4), nq = (e % (BN / 4)) * 4;
+                const int64_t nn = n0 + nq, k = k0 + kk;
+                const int n = k < K ? clamp4i(N - nn) : 0;
+                rb[i] = load4_guard(g.B + (n ? k * g.ldb + nn : 0), n);
+            }
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int i = 0; i < LA; ++i) {
+            const int e = tid + i * NT;
+            if (e < NA4) {
+                const int row = e >> 2, kq = (e & 3) * 4;
+                As[(kq + 0) * PA + row] = ra[i].x;
+                As[(kq + 1) * PA + row] = ra[i].y;
+                As[(kq + 2) * PA + row] = ra[i].z;
+                As[(kq + 3) * PA + row] = ra[i].w;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < LB; ++i) {
+            const int e = tid + i * NT;
+            if (e < NB4) {
+                const int kk = e / (BN / 4), nq = (e % (BN / 4)) * 4;
+                *reinterpret_cast<float4*>(&Bs[kk * PB + nq]) = rb[i];
+            }
         }
     };
 
@@ -122,30 +182,30 @@ __device__ __forceinline__ void tile_f32(const DGemm& g, int tm, int tn, float* 
     const int64_t nk = ceil_div(K, BK);
     load(0);
     for (int64_t kt = 0; kt < nk; ++kt) {
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < LA; ++i) {
-            const int e = tid + i * NT;
-            As[(e % BK) * PA + e / BK] = ra[i];
-        }
-#pragma unroll
-        for (int i = 0; i < LB; ++i) {
-            const int e = tid + i * NT;
-            Bs[(e / BN) * PB + e % BN] = rb[i];
-        }
+        __syncthreads();  // the previous slice (or item) is done with LDS
+        store();
         __syncthreads();
         if (kt + 1 < nk) load((kt + 1) * BK);
-#pragma unroll 16
+        FragA fa[2][GR];
+        FragB fb[2][GC];
+        auto fragment = [&](int k, int set) {
+#pragma unroll
+            for (int q = 0; q < GR; ++q)
+                fa[set][q] = *reinterpret_cast<const FragA*>(&As[k * PA + q * (BM / GR) + ty * FR]);
+#pragma unroll
+            for (int q = 0; q < GC; ++q)
+                fb[set][q] = *reinterpret_cast<const FragB*>(&Bs[k * PB + q * (BN / GC) + tx * FC]);
+        };
+        fragment(0, 0);
+#pragma unroll
         for (int k = 0; k < BK; ++k) {
-            float a[TMR], b[TNR];
-#pragma unroll
-            for (int i = 0; i < TMR; ++i) a[i] = As[k * PA + ty + i * H];
-#pragma unroll
-            for (int j = 0; j < TNR; ++j) b[j] = Bs[k * PB + tx + j * W];
+            if (k + 1 < BK) fragment(k + 1, (k + 1) & 1);
+            const float* av = reinterpret_cast<const float*>(fa[k & 1]);
+            const float* bv = reinterpret_cast<const float*>(fb[k & 1]);
 #pragma unroll
             for (int i = 0; i < TMR; ++i)
 #pragma unroll
-                for (int j = 0; j < TNR; ++j) inner[i][j] = fmaf(a[i], b[j], inner[i][j]);
+                for (int j = 0; j < TNR; ++j) inner[i][j] = fmaf(av[i], bv[j], inner[i][j]);
         }
         if ((kt + 1) % kFold == 0 || kt + 1 == nk) {
 #pragma unroll
@@ -159,11 +219,11 @@ __device__ __forceinline__ void tile_f32(const DGemm& g, int tm, int tn, float* 
     }
 #pragma unroll
     for (int i = 0; i < TMR; ++i) {
-        const int64_t m = m0 + ty + i * H;
+        const int64_t m = m0 + (i / FR) * (BM / GR) + ty * FR + (i % FR);
         if (m >= M) continue;
 #pragma unroll
         for (int j = 0; j < TNR; ++j) {
-            const int64_t n = n0 + tx + j * W;
+            const int64_t n = n0 + (j / FC) * (BN / GC) + tx * FC + (j % FC);
             if (n < N) {
                 float* c = g.C + m * g.ldc + n;
                 const float v = g.alpha * outer[i][j];
@@ -251,9 +311,9 @@ __device__ __forceinline__ void gemv_item(const DGemv& v, int chunk) {
 }
 
 // ------------------------------------------------------- persistent kernel --
-// LDS: the largest plain class footprint (32x32 with BK 64, 64x64 with BK
-// 32, 128x64 with BK 16); the split tile uses 12 KB.
-constexpr int kSmemFloats = 64 * (32 + 1) * 2 + 16;
+// LDS: the largest plain class footprint (128x64: 16 x (132 + 68) floats);
+// the split tile uses 12 KB.
+constexpr int kSmemFloats = 16 * (128 + 4) + 16 * (64 + 4) + 16;
 
 template <bool BIG>
 __global__ __launch_bounds__(NT, 2) void grouped_kernel(PArgs p) {
@@ -276,10 +336,10 @@ __global__ __launch_bounds__(NT, 2) void grouped_kernel(PArgs p) {
             w.tm = __builtin_amdgcn_readfirstlane(w.tm);
             w.tn = __builtin_amdgcn_readfirstlane(w.tn);
             switch (w.cls) {
-                case C32: tile_f32<32, 32, 2, 2, 16>(p.gemms[w.inst], w.tm, w.tn, smem); break;
-                case C64: tile_f32<64, 64, 4, 4, 16>(p.gemms[w.inst], w.tm, w.tn, smem); break;
+                case C32: tile_f32<32, 32, 2, 2>(p.gemms[w.inst], w.tm, w.tn, smem); break;
+                case C64: tile_f32<64, 64, 4, 4>(p.gemms[w.inst], w.tm, w.tn, smem); break;
                 case C128x64:
-                    if (BIG) tile_f32<128, 64, 8, 4, 16>(p.gemms[w.inst], w.tm, w.tn, smem);
+                    if (BIG) tile_f32<128, 64, 8, 4>(p.gemms[w.inst], w.tm, w.tn, smem);
                     break;
                 case CSPLIT:
                     if (p.nprod == 3)
@@ -327,14 +387,15 @@ int lg(int64_t x) {
 
 Key key_of(int mode, const omb_gemm_desc& d) { return Key{mode, lg(d.M), lg(d.N), lg(d.K)}; }
 
-// Class without a tuning entry: split mode has one class; plain picks the
-// largest tile that the smaller output dimension fills.
+// Class without a tuning entry: split mode has one class; plain takes 32x32
+// tiles when the smaller output dimension is under 48, else 64x64.  (The
+// 128x64 class needs the larger-register kernel, 2 workgroups per CU instead
+// of 3, which slows every other item of a mixed plan; at C4's shapes 64x64
+// gives the same total rate, so 128x64 is used only when tuning shows a
+// clear win.)
 int default_class(int mode, const omb_gemm_desc& d) {
     if (mode != OMB_MODE_PLAIN_F32) return CSPLIT;
-    const int64_t s = std::min(d.M, d.N);
-    if (s < 48) return C32;
-    if (s < 192) return C64;
-    return C128x64;
+    return std::min(d.M, d.N) < 48 ? C32 : C64;
 }
 
 int choose_class(int mode, const omb_gemm_desc& d) {
@@ -393,10 +454,16 @@ omb_status validate(const omb_gemm_desc* gemms, int32_t n_gemm, const omb_gemv_d
 }
 
 // Estimated seconds of one workgroup on one CU (for longest-first order).
+// Estimated seconds one workgroup spends on an item (for the longest-first
+// order): per-workgroup rates measured on MI355X with 3 workgroups per CU --
+// 32x32 16 GFLOP/s, 64x64 40, 128x64 59, split 64x64 54 (FP32-equivalent),
+// GEMV 8 GB/s -- plus about 2 us of fixed cost per item.
 double item_cost(const Item& it, const std::vector<DGemm>& g, const std::vector<DGemv>& v) {
-    if (it.cls == CGEMV) return 64.0 * (double)v[it.inst].N * 4.0 / 31e9;
-    const double rate = it.cls == CSPLIT ? 2.0e12 : 0.31e12;
-    return 2.0 * kTileM[it.cls] * kTileN[it.cls] * (double)std::max<int64_t>(g[it.inst].K, 1) / rate;
+    constexpr double kFixed = 2.0e-6;
+    if (it.cls == CGEMV) return kFixed + 64.0 * (double)v[it.inst].N * 4.0 / 8.0e9;
+    static const double rate[NCLS] = {16e9, 40e9, 59e9, 54e9, 0.0};
+    return kFixed + 2.0 * kTileM[it.cls] * kTileN[it.cls] *
+                        (double)std::max<int64_t>(g[it.inst].K, 1) / rate[it.cls];
 }
 
 template <typename T>
@@ -701,6 +768,9 @@ extern "C" omb_status omb_autotune_group(const omb_gemm_desc* gemms, int32_t n_g
             float ms = 1e30f;
             st = time_class(mode, rep.M, rep.N, rep.K, c, batch, A, B, C, s, &ms);
             if (st != OMB_OK) break;
+            // 128x64 must beat the others by 20%: it costs the rest of a
+            // mixed plan one workgroup per CU (see default_class)
+            if (c == C128x64) ms *= 1.2f;
             if (ms < best_ms) {
                 best_ms = ms;
                 best = c;
