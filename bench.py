@@ -83,7 +83,7 @@ def main():
                     help="seconds of host time for the CPU baseline sample")
     ap.add_argument("--legs", default="all",
                     help="comma list of legs beside the headline: plain, errors, gemv, dot, "
-                         "grouped, serving, cpu (default all)")
+                         "level2, grouped, serving, cpu (default all)")
     ap.add_argument("--serve-rate", type=float, default=float(os.environ.get("OMB_SERVE_RATE", 32000)),
                     help="serving leg: Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-count", type=int, default=int(os.environ.get("OMB_SERVE_COUNT", 3000)),
@@ -96,7 +96,7 @@ def main():
               "(the multi-GPU sharding is a design, not a measured path)", file=sys.stderr)
         sys.exit(2)
 
-    legs = {"plain", "errors", "gemv", "dot", "grouped", "serving", "cpu"}
+    legs = {"plain", "errors", "gemv", "dot", "level2", "grouped", "serving", "cpu"}
     if args.legs != "all":
         legs = {x.strip() for x in args.legs.split(",") if x.strip()}
     if args.no_errors:
@@ -175,6 +175,21 @@ def main():
                          "fp32_equiv_tflops": flop / t_k / 1e12,
                          "frac_of_bf16_peak": nprod * flop / t_k / PEAK_MFMA_BF16}
     t_k6 = kernels["sgemm_bf16x3_kernel<6> (pre-packed operands)"]["ms"] / 1e3
+    # The same kernel on all-zero operands: identical instructions and memory
+    # traffic, no switching activity in the MFMA datapath.  The gap to the
+    # random-operand time is the power-limited, data-dependent MFMA rate.
+    with torch.cuda.stream(stream):
+        Ap.zero_()
+        Bp.zero_()
+    sync()
+    fn = lambda: lib.omb_sgemm_packed(n, n, n, 1.0, Ap.data_ptr(), Bp.data_ptr(), 0.0,
+                                      C.data_ptr(), n, 1, 6, sp)
+    fn()
+    t_z = timer.run(fn, max(3, args.steps // 2))
+    kernels["sgemm_bf16x3_kernel<6> (zero operands)"] = {
+        "ms": t_z * 1e3, "mfma_tflops": 6 * flop / t_z / 1e12,
+        "frac_of_bf16_peak": 6 * flop / t_z / PEAK_MFMA_BF16,
+        "random_over_zero_time": t_k6 / t_z}
     kernels["split_pack_kernel (8192x8192)"] = {
         "ms": t_pack * 1e3, "gbps": (4 + 6) * n * n / t_pack / 1e9,
         "frac_of_hbm_peak": (4 + 6) * n * n / t_pack / PEAK_HBM}
@@ -247,6 +262,43 @@ def main():
         del dx, dy
     sync()
 
+    # ------------------------------------------- outer product / transpose --
+    level2 = {}
+    if "level2" in legs:
+        log("sger and transpose 16384^2 (seed 1)")
+        m = 16384
+        with torch.cuda.stream(stream):
+            LA = torch.empty((m, m), device=dev)
+            LC = torch.empty((m, m), device=dev)
+            lx = torch.empty(m, device=dev)
+            ly = torch.empty(m, device=dev)
+        for t, sid in ((LA, 0), (lx, 1), (ly, 2)):
+            pkg.fill_uniform(t, 1, sid, stream=sp)
+        fn = lambda: lib.omb_sger(m, m, 1.0, lx.data_ptr(), ly.data_ptr(), LA.data_ptr(), m, 1, sp)
+        fn()
+        t_g = timer.run(fn, max(10, args.steps))
+        nbytes = 8.0 * m * m + 4.0 * (2 * m)
+        level2["sger_row"] = {"us": t_g * 1e6, "gbps": nbytes / t_g / 1e9,
+                              "frac_of_hbm_peak": nbytes / t_g / PEAK_HBM, "bytes": nbytes}
+        fn = lambda: lib.omb_stranspose(m, m, LA.data_ptr(), m, 1, LC.data_ptr(), m, 1, sp)
+        fn()
+        t_t = timer.run(fn, max(10, args.steps))
+        nbytes = 8.0 * m * m
+        level2["transpose_out_of_place"] = {"us": t_t * 1e6, "gbps": nbytes / t_t / 1e9,
+                                            "frac_of_hbm_peak": nbytes / t_t / PEAK_HBM,
+                                            "bytes": nbytes}
+        fn = lambda: lib.omb_stranspose_inplace(m, LC.data_ptr(), m, sp)
+        fn()
+        t_i = timer.run(fn, max(10, args.steps))
+        level2["transpose_in_place"] = {"us": t_i * 1e6, "gbps": nbytes / t_i / 1e9,
+                                        "frac_of_hbm_peak": nbytes / t_i / PEAK_HBM,
+                                        "bytes": nbytes}
+        kernels["ger_kernel (16384^2, row-major C)"] = level2["sger_row"]
+        kernels["transpose_tiles_kernel (16384^2)"] = level2["transpose_out_of_place"]
+        kernels["transpose_inplace_kernel (16384^2)"] = level2["transpose_in_place"]
+        del LA, LC
+        sync()
+
     # ------------------------------------------------ grouped and serving --
     grouped, serving = None, None
     if "grouped" in legs:
@@ -295,6 +347,7 @@ def main():
         "errors": errors,
         "gemv": gemv,
         "dot": dot,
+        "outer_transpose": level2,
         "grouped": grouped,
         "serving": serving,
         "kernels": kernels,

@@ -92,6 +92,24 @@ omb_status omb_sgemv(int64_t M, int64_t N, float alpha,
 omb_status omb_sdot(int64_t n, const float* x, const float* y, float* result,
                      omb_stream_t stream);
 
+/* --------------------------------------------------------- outer product ----
+ * C += alpha * x * y^T, C M x N in either storage order (rsC, csC); every
+ * element is fma(alpha * x[i], y[j], C[i][j]).  Replaces
+ * amt::outer_prod_helper (include/outer.hpp:15-48; alpha = 1 there). */
+omb_status omb_sger(int64_t M, int64_t N, float alpha, const float* x, const float* y,
+                    float* C, int64_t rsC, int64_t csC, omb_stream_t stream);
+
+/* ------------------------------------------------------------- transpose ----
+ * Out of place: C (N x M, strides rsC, csC) = A^T (A M x N, strides rsA,
+ * csA); A and C must not overlap.  Replaces amt::transpose_helper(...,
+ * tag::outplace) (include/trans.hpp:23-45).
+ * In place: the n x n matrix at A (rows ld apart, either storage order) is
+ * replaced by its transpose.  Replaces transpose_helper(..., tag::inplace)
+ * (include/trans.hpp:47-71). */
+omb_status omb_stranspose(int64_t M, int64_t N, const float* A, int64_t rsA, int64_t csA,
+                          float* C, int64_t rsC, int64_t csC, omb_stream_t stream);
+omb_status omb_stranspose_inplace(int64_t n, float* A, int64_t ld, omb_stream_t stream);
+
 /* ------------------------------------------------------- input filling ----
  * Counter-based uniform[-1,1) generator: element i of (seed, stream_id) is
  * top24(splitmix64(key(seed, stream_id), offset + i)) * 2^-23 - 1 (exact). */

@@ -4,6 +4,8 @@
 //   amt::mtv_helper      include/mtv.hpp:15-22     first_order (column-major A): c += A*b
 //                        include/mtv.hpp:72-79     last_order  (row-major A):    c  = A*b
 //   amt::dot_prod_helper include/dot.hpp:143-149   *c = sum a[i]*b[i]
+//   amt::outer_prod_helper include/outer.hpp:15-48 C lines += a * b[i]
+//   amt::transpose_helper include/trans.hpp:23-71  C = A^T (out of place / square in place)
 //
 // Same template signatures, argument meanings (n* = extents {rows, cols},
 // w* = strides {row stride, column stride}) and accumulate / overwrite
@@ -196,6 +198,78 @@ void dot_prod_helper(ValueType* c, ValueType const* a, SizeType const* na, Value
         dropin::staged<float> R(c, 1, false, true, o.stream);
         dropin::check(omb_sdot(n, X.get(), Y.get(), R.get(), o.stream), "amt::dot_prod_helper");
         R.finish();
+    }
+}
+
+// Tags of transpose_helper (the reference's amt::tag, include/utils.hpp:71-73).
+namespace tag {
+struct inplace {};
+struct outplace {};
+}  // namespace tag
+
+// C line i (WC = max(wc) apart) += a * b[i] for i < nb[0]*nb[1], lines of
+// na[0]*na[1] elements (include/outer.hpp:15-48): a rank-1 update of the
+// lines viewed as a row-major NB x NA matrix.
+template <class ValueType, class SizeType>
+void outer_prod_helper(ValueType* c, SizeType const* nc, SizeType const* wc, ValueType const* a,
+                       SizeType const* na, ValueType const* b, SizeType const* nb,
+                       int max_threads) {
+    (void)nc;
+    (void)max_threads;
+    if constexpr (!std::is_same_v<ValueType, float>) {
+        throw std::runtime_error("amt::outer_prod_helper: only float is implemented on the GPU");
+    } else {
+        const int64_t NA = static_cast<int64_t>(na[0] * na[1]);
+        const int64_t NB = static_cast<int64_t>(nb[0] * nb[1]);
+        const int64_t WC = static_cast<int64_t>(wc[0] > wc[1] ? wc[0] : wc[1]);
+        auto& o = dropin::settings();
+        dropin::staged<const float> X(b, static_cast<std::size_t>(NB), true, false, o.stream);
+        dropin::staged<const float> Y(a, static_cast<std::size_t>(NA), true, false, o.stream);
+        dropin::staged<float> C(c, dropin::span(NB, NA, WC, 1), true, true, o.stream);
+        dropin::check(omb_sger(NB, NA, 1.f, X.get(), Y.get(), C.get(), WC, 1, o.stream),
+                      "amt::outer_prod_helper");
+        C.finish();
+    }
+}
+
+// Out of place: C = A^T, A extents na, strides wa; C strides wc
+// (include/trans.hpp:23-45).
+template <class ValueType, class SizeType>
+void transpose_helper(ValueType* c, SizeType const* nc, SizeType const* wc, ValueType const* a,
+                      SizeType const* na, SizeType const* wa, tag::outplace) {
+    (void)nc;
+    if constexpr (!std::is_same_v<ValueType, float>) {
+        throw std::runtime_error("amt::transpose_helper: only float is implemented on the GPU");
+    } else {
+        const int64_t M = static_cast<int64_t>(na[0]), N = static_cast<int64_t>(na[1]);
+        auto& o = dropin::settings();
+        dropin::staged<const float> A(a, dropin::span(M, N, wa[0], wa[1]), true, false, o.stream);
+        dropin::staged<float> C(c, dropin::span(N, M, wc[0], wc[1]), false, true, o.stream);
+        dropin::check(omb_stranspose(M, N, A.get(), wa[0], wa[1], C.get(), wc[0], wc[1], o.stream),
+                      "amt::transpose_helper");
+        C.finish();
+    }
+}
+
+// In place: the square matrix at a (na[0] x na[0], rows max(wa) apart) is
+// replaced by its transpose (include/trans.hpp:47-71; c == a there).
+template <class ValueType, class SizeType>
+void transpose_helper(ValueType* c, SizeType const* nc, SizeType const* wc, ValueType* a,
+                      SizeType const* na, SizeType const* wa, tag::inplace) {
+    (void)c;
+    (void)nc;
+    (void)wc;
+    if constexpr (!std::is_same_v<ValueType, float>) {
+        throw std::runtime_error("amt::transpose_helper: only float is implemented on the GPU");
+    } else {
+        const int64_t n = static_cast<int64_t>(na[0]);
+        if (na[1] != na[0])
+            throw std::runtime_error("amt::transpose_helper: in-place transpose needs a square matrix");
+        const int64_t ld = static_cast<int64_t>(wa[0] > wa[1] ? wa[0] : wa[1]);
+        auto& o = dropin::settings();
+        dropin::staged<float> A(a, dropin::span(n, n, ld, 1), true, true, o.stream);
+        dropin::check(omb_stranspose_inplace(n, A.get(), ld, o.stream), "amt::transpose_helper");
+        A.finish();
     }
 }
 

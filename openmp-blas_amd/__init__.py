@@ -12,6 +12,8 @@ tensors that live on the GPU:
     mtv(c, a, b)()        column-major A: c += A*b ; row-major A: c = A*b
     vtm(c, a, b)()        c = A^T b (row-major A: accumulate, col-major: overwrite)
     dot_prod(c, a, b)()   c[0] = sum a*b
+    outer_prod(c, a, b)() C += a b^T
+    transpose(c, a)()     C = A^T   (transpose(a)() in place, square A)
 
 A tensor's layout is read from its strides: row-major ("last_order") when
 the column stride is 1, column-major ("first_order") when the row stride is 1.
@@ -110,6 +112,9 @@ _SIGS = {
                                 _VP]),
     "omb_sgemv": (_I32, [_I64, _I64, _F32, _VP, _I64, _I64, _VP, _F32, _VP, _VP]),
     "omb_sdot": (_I32, [_I64, _VP, _VP, _VP, _VP]),
+    "omb_sger": (_I32, [_I64, _I64, _F32, _VP, _VP, _VP, _I64, _I64, _VP]),
+    "omb_stranspose": (_I32, [_I64, _I64, _VP, _I64, _I64, _VP, _I64, _I64, _VP]),
+    "omb_stranspose_inplace": (_I32, [_I64, _VP, _I64, _VP]),
     "omb_fill_uniform": (_I32, [_VP, _I64, _U64, _U64, _I64, _VP]),
     "omb_check_dgemm": (_I32, [_I64, _I64, _I64, _VP, _I64, _I64, _VP, _I64, _I64, _VP, _I64,
                                _I64, _VP, _VP]),
@@ -187,6 +192,18 @@ def sgemv(M, N, alpha, A, rsA, csA, x, beta, y, stream=None) -> int:
 
 def sdot(n, x, y, result, stream=None) -> int:
     return lib().omb_sdot(n, x, y, result, _stream(stream))
+
+
+def sger(M, N, alpha, x, y, C, rsC, csC, stream=None) -> int:
+    return lib().omb_sger(M, N, alpha, x, y, C, rsC, csC, _stream(stream))
+
+
+def stranspose(M, N, A, rsA, csA, C, rsC, csC, stream=None) -> int:
+    return lib().omb_stranspose(M, N, A, rsA, csA, C, rsC, csC, _stream(stream))
+
+
+def stranspose_inplace(n, A, ld, stream=None) -> int:
+    return lib().omb_stranspose_inplace(n, A, ld, _stream(stream))
 
 
 # ------------------------------------------------------ torch conveniences --
@@ -291,6 +308,54 @@ def dot_prod(c, a, b, num_threads=None, stream=None):
 
     def run():
         check(sdot(n, _ptr(a), _ptr(b), _ptr(c), stream))
+
+    return run
+
+
+def outer_prod(c, a, b, num_threads=None, stream=None):
+    """amt::outer_prod (include/outer.hpp:51-122): C += a b^T for vectors a
+    (length M) and b (length N) and an M x N matrix C of either layout."""
+    if not (is_vector(a) and is_vector(b)):
+        raise RuntimeError("amt::outer_prod: both tensor must be vector")
+    M, N = _vec_len(a), _vec_len(b)
+    if not is_matrix(c) or c.shape[0] != M or c.shape[1] != N:
+        raise RuntimeError("amt::outer_prod: dimension mismatch")
+    rs, cs = _strides(c)
+
+    def run():
+        check(sger(M, N, 1.0, _ptr(a), _ptr(b), _ptr(c), rs, cs, stream))
+
+    return run
+
+
+def transpose(c, a=None, num_threads=None, stream=None):
+    """amt::transpose (include/trans.hpp:82-166).  transpose(c, a)(): C = A^T
+    out of place (C is N x M for an M x N A, any layouts); transpose(a)(): the
+    square matrix a is transposed in place."""
+    if a is None:
+        a = c
+        if not is_matrix(a):
+            raise RuntimeError("amt::transpose: a must be a matrix")
+        if a.shape[0] != a.shape[1]:
+            raise RuntimeError("amt::transpose: in-place transpose needs a square matrix")
+        rs, cs = _strides(a)
+        if min(rs, cs) != 1:
+            raise RuntimeError("amt::transpose: one stride must be 1")
+        n, ld = a.shape[0], max(rs, cs)
+
+        def run_inplace():
+            check(stranspose_inplace(n, _ptr(a), ld, stream))
+
+        return run_inplace
+    if not (is_matrix(a) and is_matrix(c)):
+        raise RuntimeError("amt::transpose: a and c must be the matrices")
+    if a.shape[0] != c.shape[1] or a.shape[1] != c.shape[0]:
+        raise RuntimeError("amt::transpose: dimension mismatch")
+    M, N = a.shape
+    (rsA, csA), (rsC, csC) = _strides(a), _strides(c)
+
+    def run():
+        check(stranspose(M, N, _ptr(a), rsA, csA, _ptr(c), rsC, csC, stream))
 
     return run
 

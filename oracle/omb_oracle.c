@@ -378,6 +378,91 @@ void oracle_smtv_last(float* y, const float* a, int64_t M, int64_t N, const int6
 }
 
 /* ---------------------------------------------------------------------------
+ * outer product: include/outer.hpp:15-48 (outer_prod_helper) with
+ * simd_loop<OUTER> (include/simd_loop.hpp:38-50): line i of C (WC apart)
+ * gets c[k] += a[k] * b[i] for k < NA; lines in parallel when NB > 256.
+ * The closure (include/outer.hpp:104-120) passes first-order C as columns
+ * (wc = {1, M}) and last-order C as rows with a and b swapped.
+ * ------------------------------------------------------------------------- */
+void oracle_souter_lines(float* c, int64_t WC, const float* a, int64_t NA, const float* b,
+                         int64_t NB, int nthreads) {
+    int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(static) if (NB > 256) num_threads(nt)
+    for (int64_t i = 0; i < NB; ++i) {
+        float* cj = c + i * WC;
+        const float cst = b[i];
+#pragma omp simd
+        for (int64_t k = 0; k < NA; ++k) cj[k] += a[k] * cst;
+    }
+}
+
+void oracle_souter(float* c, int64_t M, int64_t N, int last_order, const float* a,
+                   const float* b, int nthreads) {
+    if (M <= 0 || N <= 0) return;
+    if (last_order)
+        oracle_souter_lines(c, N, b, N, a, M, nthreads);
+    else
+        oracle_souter_lines(c, M, a, M, b, N, nthreads);
+}
+
+/* ---------------------------------------------------------------------------
+ * transpose: include/trans.hpp:11-71 with simd_loop<TRANS>
+ * (include/simd_loop.hpp:193-240).  Block edge = sqrt_pow_of_two(L1 bytes /
+ * (2 * sizeof(float))) (include/utils.hpp:49-58).
+ * ------------------------------------------------------------------------- */
+int64_t oracle_transpose_block(int64_t l1_bytes) {
+    int64_t n = l1_bytes > 0 ? l1_bytes : sysconf(_SC_LEVEL1_DCACHE_SIZE);
+    if (n <= 0) n = 32768;
+    n /= 2 * (int64_t)sizeof(float);
+    int64_t p = 0;
+    n >>= 1;
+    while (n) {
+        n >>= 1;
+        ++p;
+    }
+    p >>= 1;
+    return (int64_t)1 << p;
+}
+
+void oracle_stranspose_out(float* c, const int64_t* wc, const float* a, const int64_t* na,
+                           const int64_t* wa, int64_t block) {
+    if (block <= 0) block = oracle_transpose_block(0);
+    for (int64_t i = 0; i < na[0]; i += block) {
+        const int64_t ib = imin(na[0] - i, block);
+        for (int64_t j = 0; j < na[1]; j += block) {
+            const int64_t jb = imin(na[1] - j, block);
+            float* ck = c + i * wc[1] + j * wc[0];
+            const float* ak = a + i * wa[0] + j * wa[1];
+            for (int64_t ii = 0; ii < ib; ++ii)
+                for (int64_t jj = 0; jj < jb; ++jj)
+                    ck[ii * wc[1] + jj * wc[0]] = ak[ii * wa[0] + jj * wa[1]];
+        }
+    }
+}
+
+void oracle_stranspose_in(float* a, int64_t n, int64_t block) {
+    if (block <= 0) block = oracle_transpose_block(0);
+    /* the closure's strides (include/trans.hpp:158-163): wc = {1, n}, wa = {1, n} */
+    for (int64_t i = 0; i < n; i += block) {
+        const int64_t ib = imin(n - i, block);
+        for (int64_t j = i; j < n; j += block) {
+            const int64_t jb = imin(n - j, block);
+            const int same = i == j;
+            float* ck = a + i * n + j;  /* c + i*wc[1] + j*wc[0] */
+            float* ak = a + i + j * n;  /* a + i*wa[0] + j*wa[1] */
+            for (int64_t ii = 0; ii < ib && jb > ii * same; ++ii)
+                for (int64_t jj = ii * same; jj < jb; ++jj) {
+                    float* x = ck + ii * n + jj;
+                    float* y = ak + ii + jj * n;
+                    const float t = *x;
+                    *x = *y;
+                    *y = t;
+                }
+        }
+    }
+}
+
+/* ---------------------------------------------------------------------------
  * dot: include/dot.hpp:16-53 (section two / three loops) and :143-174.
  * The reference changes the global OpenMP team size in section two
  * (:166-168); here the team size is a num_threads clause instead.

@@ -20,6 +20,11 @@
  *   - oracle_smtv/_svtm     : integer-exact brute force (test/test.mtv.cpp,
  *                             test/test.vtm.cpp sizes 2..511).
  *   - oracle_sdot           : closed form s(s+1)(2s+1)/6 (test/test.dot_prod.cpp).
+ *   - oracle_souter*        : integer-exact brute force on the sizes of
+ *                             test/test.outer_prod.cpp (2..31, both orders).
+ *   - oracle_stranspose_*   : element-for-element brute force on the sizes of
+ *                             test/test.trans.cpp (2..31, out of and in place)
+ *                             and the block edge against the doc laptop L1.
  *   - omb_rng_*             : splitmix64 known-answer vectors (published
  *                             reference outputs of splitmix64 for seed 0).
  */
@@ -94,6 +99,24 @@ void oracle_smtv_last(float* y, const float* a, int64_t M, int64_t N,
 /* dot_prod_helper (include/dot.hpp:143-174): returns sum a[i]*b[i]. */
 float oracle_sdot(const float* a, const float* b, int64_t n,
                   const oracle_blocking_t* blk, int nthreads);
+
+/* outer_prod_helper (include/outer.hpp:15-48): NB lines of C, WC apart,
+ * line i += a * b[i].  oracle_souter: the closure's C (M x N) += a b^T for
+ * either storage order (include/outer.hpp:104-120). */
+void oracle_souter_lines(float* c, int64_t WC, const float* a, int64_t NA, const float* b,
+                         int64_t NB, int nthreads);
+void oracle_souter(float* c, int64_t M, int64_t N, int last_order, const float* a,
+                   const float* b, int nthreads);
+
+/* transpose_helper (include/trans.hpp:23-71): out of place C = A^T with
+ * C strides wc, A extents na and strides wa; in place for a square
+ * column-major n x n matrix.  block <= 0: the reference's edge for this
+ * host's L1; oracle_transpose_block(l1_bytes) gives it for any L1 size
+ * (l1_bytes <= 0: this host's). */
+int64_t oracle_transpose_block(int64_t l1_bytes);
+void oracle_stranspose_out(float* c, const int64_t* wc, const float* a, const int64_t* na,
+                           const int64_t* wa, int64_t block);
+void oracle_stranspose_in(float* a, int64_t n, int64_t block);
 
 /* Counter-based input generator shared bit-for-bit with the device fill
  * kernel: value(seed, stream, i) = splitmix64 finaliser chain. */

@@ -1,7 +1,8 @@
 // Tests of the C++ drop-in layer (ompblas_dropin.hpp) in the shape of the
 // reference's Catch2 suites: test/test.mtm.cpp (8 layout combinations,
 // sizes 2..31), test/test.mtv.cpp (first / last order, sizes 2..511 and
-// 32 Ki), test/test.dot_prod.cpp (1..s closed form).  Inputs are
+// 32 Ki), test/test.dot_prod.cpp (1..s closed form), test/test.outer_prod.cpp
+// and test/test.trans.cpp (sizes 2..31, every layout, in and out of place).  Inputs are
 // rand() % 100 as in test/test_utils.hpp; the expected values come from
 // integer brute force (exact while sums stay below 2^24) instead of BLIS,
 // and from a relative bound for the 32 Ki case (the reference's Approx).
@@ -12,6 +13,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -230,6 +232,96 @@ void test_dot(bool quick) {
     (void)omb_device_free(r_dev);
 }
 
+// test/test.outer_prod.cpp: C += a b^T through the closure's helper calls
+// (include/outer.hpp:104-120): first-order C as columns, last-order C as
+// rows with a and b swapped.
+void outer_case(bool first, std::size_t m, std::size_t n, bool device) {
+    Mat C(m, n, first);
+    std::vector<float> a(m), b(n);
+    rand_fill(a);
+    rand_fill(b);
+    rand_fill(C.v);
+    std::vector<int64_t> want(m * n);
+    for (std::size_t i = 0; i < m; ++i)
+        for (std::size_t j = 0; j < n; ++j)
+            want[i * n + j] = static_cast<int64_t>(C.at(i, j)) +
+                              static_cast<int64_t>(a[i]) * static_cast<int64_t>(b[j]);
+    const std::size_t na[2] = {1, m}, nb[2] = {1, n};
+    const std::size_t wc_first[2] = {1, m}, wc_last[2] = {n, 1};
+    auto call = [&](float* cp, const float* ap, const float* bp) {
+        if (first)
+            amt::outer_prod_helper(cp, C.n, wc_first, ap, na, bp, nb, 8);
+        else
+            amt::outer_prod_helper(cp, C.n, wc_last, bp, nb, ap, na, 8);
+    };
+    if (device) {
+        Dev dc(C.v), da(a), db(b);
+        call(dc.p, da.p, db.p);
+        C.v = dc.get();
+    } else {
+        call(C.v.data(), a.data(), b.data());
+    }
+    for (std::size_t i = 0; i < m; ++i)
+        for (std::size_t j = 0; j < n; ++j)
+            CHECK(static_cast<int64_t>(C.at(i, j)) == want[i * n + j],
+                  "outer %s %zux%zu %s (%zu,%zu)", first ? "first" : "last", m, n,
+                  device ? "device" : "host", i, j);
+}
+
+// test/test.trans.cpp: out of place for every layout pair, and in place.
+void transpose_case(bool fa, bool fc, std::size_t m, std::size_t n, bool device) {
+    Mat A(m, n, fa), C(n, m, fc);
+    rand_fill(A.v);
+    std::fill(C.v.begin(), C.v.end(), -1.f);
+    if (device) {
+        Dev da(A.v), dc(C.v);
+        amt::transpose_helper(dc.p, C.n, C.w, static_cast<const float*>(da.p), A.n, A.w,
+                              amt::tag::outplace{});
+        C.v = dc.get();
+    } else {
+        amt::transpose_helper(C.v.data(), C.n, C.w, static_cast<const float*>(A.v.data()), A.n,
+                              A.w, amt::tag::outplace{});
+    }
+    for (std::size_t i = 0; i < m; ++i)
+        for (std::size_t j = 0; j < n; ++j)
+            CHECK(C.at(j, i) == A.at(i, j), "transpose %c%c %zux%zu %s (%zu,%zu)", fa ? 'F' : 'L',
+                  fc ? 'F' : 'L', m, n, device ? "device" : "host", i, j);
+}
+
+void transpose_inplace_case(std::size_t n, bool device) {
+    Mat A(n, n, true);
+    rand_fill(A.v);
+    const std::vector<float> orig = A.v;
+    const std::size_t wc[2] = {1, n}, wa[2] = {1, n};
+    if (device) {
+        Dev da(A.v);
+        amt::transpose_helper(da.p, A.n, wc, da.p, A.n, wa, amt::tag::inplace{});
+        A.v = da.get();
+    } else {
+        amt::transpose_helper(A.v.data(), A.n, wc, A.v.data(), A.n, wa, amt::tag::inplace{});
+    }
+    for (std::size_t i = 0; i < n; ++i)
+        for (std::size_t j = 0; j < n; ++j)
+            CHECK(A.v[i + j * n] == orig[j + i * n], "transpose in place n=%zu %s (%zu,%zu)", n,
+                  device ? "device" : "host", i, j);
+}
+
+void test_outer_transpose(bool quick) {
+    for (int first = 0; first < 2; ++first) {
+        for (std::size_t sz = 2; sz < 32; sz += quick ? 7 : 1) outer_case(first, sz, sz, false);
+        outer_case(first, 3, 70, false);
+        outer_case(first, 129, 5, true);
+    }
+    for (int l = 0; l < 4; ++l) {
+        for (std::size_t sz = 2; sz < 32; sz += quick ? 7 : 1) transpose_case(l & 1, l & 2, sz, sz, false);
+        transpose_case(l & 1, l & 2, 70, 3, false);
+        transpose_case(l & 1, l & 2, 129, 200, true);
+    }
+    for (std::size_t sz = 2; sz < 32; sz += quick ? 7 : 1) transpose_inplace_case(sz, false);
+    transpose_inplace_case(65, true);
+    transpose_inplace_case(300, false);
+}
+
 void test_errors() {
     // only float is computed
     double c = 0, a[4] = {1, 2, 3, 4};
@@ -273,6 +365,8 @@ int main(int argc, char** argv) {
             test_mtv_large(false);
             std::printf("mtv 32Ki: %d checks, %d failures\n", g_checks, g_failures);
         }
+        test_outer_transpose(quick);
+        std::printf("outer/transpose: %d checks, %d failures\n", g_checks, g_failures);
         test_errors();
         amt::dropin::settings().gemm_mode = OMB_MODE_SPLIT_BF16;
         for (std::size_t sz : {5, 31}) mtm_case<amt::layout::last_order>(false, true, false, sz, false, "LFL split");

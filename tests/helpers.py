@@ -76,6 +76,11 @@ def oracle():
         o.omb_rng_fill_uniform.argtypes = [_VP, _I64, ctypes.c_uint64, ctypes.c_uint64, _I64]
         o.oracle_rand100_fill.argtypes = [_VP, _I64]
         o.oracle_srand.argtypes = [ctypes.c_uint]
+        o.oracle_souter.argtypes = [_VP, _I64, _I64, ctypes.c_int, _VP, _VP, ctypes.c_int]
+        o.oracle_transpose_block.restype = _I64
+        o.oracle_transpose_block.argtypes = [_I64]
+        o.oracle_stranspose_out.argtypes = [_VP, _I64P, _VP, _I64P, _I64P, _I64]
+        o.oracle_stranspose_in.argtypes = [_VP, _I64, _I64]
         _oracle = o
     return _oracle
 
@@ -160,6 +165,26 @@ def oracle_dot(a: np.ndarray, b: np.ndarray, blk: Blocking | None = None,
                nthreads: int = 0) -> float:
     return float(oracle().oracle_sdot(_ptr(a), _ptr(b), a.size,
                                       ctypes.byref(blk) if blk is not None else None, nthreads))
+
+
+def oracle_outer(c: np.ndarray, a: np.ndarray, b: np.ndarray, nthreads: int = 0) -> None:
+    """C += a b^T through the reference closure's path for C's storage order."""
+    M, N = c.shape
+    assert c.flags.c_contiguous or c.flags.f_contiguous
+    last = 1 if c.flags.c_contiguous and not (c.flags.f_contiguous and M > 1) else 0
+    oracle().oracle_souter(_ptr(c), M, N, last, _ptr(a), _ptr(b), nthreads)
+
+
+def oracle_transpose(c: np.ndarray, a: np.ndarray, block: int = 0) -> None:
+    """C = A^T out of place (any strides; c is N x M for an M x N a)."""
+    oracle().oracle_stranspose_out(_ptr(c), _arr(*_elem_strides(c)), _ptr(a), _arr(*a.shape),
+                                   _arr(*_elem_strides(a)), block)
+
+
+def oracle_transpose_inplace(a: np.ndarray, block: int = 0) -> None:
+    """Square column-major a transposed in place."""
+    assert a.shape[0] == a.shape[1] and a.flags.f_contiguous
+    oracle().oracle_stranspose_in(_ptr(a), a.shape[0], block)
 
 
 def rand100(shape, order="C") -> np.ndarray:

@@ -174,3 +174,32 @@ def test_cpp_dropin_program_is_built_against_the_library(ompblas):
     assert os.path.exists(path)
     deps = subprocess.run(["ldd", path], capture_output=True, text=True).stdout
     assert "libompblas.so" in deps and "oracle" not in deps
+
+
+def test_ger_and_transpose_reject_bad_arguments(ompblas):
+    lib = ompblas.lib()
+    assert lib.omb_sger(-1, 4, 1.0, 16, 16, 16, 4, 1, None) == -1
+    assert lib.omb_sger(4, 4, 1.0, None, 16, 16, 4, 1, None) == -1
+    assert lib.omb_sger(4, 4, 1.0, 16, 16, 16, 2, 1, None) == -1          # ld < N
+    assert lib.omb_sger(4, 4, 1.0, 16, 16, 16, 8, 2, None) == -2          # no unit stride
+    assert lib.omb_sger(0, 4, 1.0, None, None, None, 4, 1, None) == 0     # empty: no device call
+    assert lib.omb_stranspose(4, 4, None, 4, 1, 16, 4, 1, None) == -1
+    assert lib.omb_stranspose(4, 5, 16, 5, 1, 4096, 3, 1, None) == -1     # C is 5 x 4: ld 3 < 4
+    assert lib.omb_stranspose(4, 4, 4096, 4, 1, 4096 + 8, 4, 1, None) == -1  # overlap
+    assert "overlap" in ompblas.last_error()
+    assert lib.omb_stranspose_inplace(8, 16, 4, None) == -1               # ld < n
+    assert lib.omb_stranspose_inplace(0, None, 0, None) == 0
+
+
+def test_outer_and_transpose_mirror_shape_errors(ompblas):
+    """amt::outer_prod / amt::transpose throw at closure creation
+    (include/outer.hpp:75-99, include/trans.hpp:97-115)."""
+    import torch
+    with pytest.raises(RuntimeError, match="dimension mismatch"):
+        ompblas.outer_prod(torch.zeros(3, 4), torch.zeros(3), torch.zeros(5))
+    with pytest.raises(RuntimeError, match="must be vector"):
+        ompblas.outer_prod(torch.zeros(3, 4), torch.zeros(3, 2), torch.zeros(4))
+    with pytest.raises(RuntimeError, match="dimension mismatch"):
+        ompblas.transpose(torch.zeros(3, 4), torch.zeros(3, 4))
+    with pytest.raises(RuntimeError, match="square"):
+        ompblas.transpose(torch.zeros(3, 4))

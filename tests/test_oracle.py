@@ -270,3 +270,61 @@ def test_generator_host_twins_agree():
     assert c.min() >= -1.0 and c.max() < 1.0
     # exact 24-bit lattice: (x + 1) * 2^23 is an integer
     assert np.all(np.mod((c.astype(np.float64) + 1.0) * 2 ** 23, 1.0) == 0.0)
+
+
+# ----------------------------------------------------- outer / transpose --
+@pytest.mark.parametrize("order", ["F", "C"])
+def test_outer_integer_exact(order):
+    """test/test.outer_prod.cpp:29-182: C += a b^T, sizes 2..31 (square) plus
+    rectangular shapes, both storage orders; brute force in int64."""
+    shapes = [(s, s) for s in range(2, 32)] + [(3, 17), (40, 5), (1, 9), (300, 257)]
+    for m, n in shapes:
+        a, b = rand100((m,)), rand100((n,))
+        c = rand100((m, n), order)
+        want = c.astype(np.int64) + a.astype(np.int64)[:, None] * b.astype(np.int64)[None, :]
+        helpers.oracle_outer(c, a, b)
+        assert np.array_equal(c.astype(np.int64), want), (order, m, n)
+
+
+def test_outer_float_is_one_rounding_per_element():
+    rng = np.random.default_rng(7)
+    a = rng.uniform(-1, 1, 300).astype(np.float32)
+    b = rng.uniform(-1, 1, 200).astype(np.float32)
+    c0 = rng.uniform(-1, 1, (300, 200)).astype(np.float32)
+    c = c0.copy()
+    helpers.oracle_outer(c, a, b)
+    exact = c0.astype(np.float64) + a.astype(np.float64)[:, None] * b.astype(np.float64)[None, :]
+    # one fused multiply-add, or a product and a sum: within 1.5 ulp of the exact value
+    assert np.all(np.abs(c - exact) <= 1.5 * np.spacing(np.abs(exact).astype(np.float32)))
+
+
+def test_transpose_block_edge_formula():
+    """transpose_partition::block() = sqrt_pow_of_two(L1 / (2 sizeof(float)))
+    (include/trans.hpp:11-21, include/utils.hpp:49-58), worked by hand."""
+    o = helpers.oracle()
+    assert o.oracle_transpose_block(32768) == 64     # 4096 -> 2^(12 >> 1)
+    assert o.oracle_transpose_block(16384) == 32     # 2048 -> 2^(11 >> 1)
+    assert o.oracle_transpose_block(131072) == 128   # 16384 -> 2^(14 >> 1)
+    assert o.oracle_transpose_block(49152) == 64     # 6144 -> 2^(12 >> 1)
+
+
+@pytest.mark.parametrize("ao,co", [("C", "C"), ("C", "F"), ("F", "C"), ("F", "F")])
+def test_transpose_out_of_place_exact(ao, co):
+    """test/test.trans.cpp:11-42: C = A^T, sizes 2..31 plus rectangular and
+    multi-block shapes, every storage-order pair; small blocks exercise the
+    edges."""
+    shapes = [(s, s) for s in range(2, 32)] + [(3, 70), (70, 3), (129, 65)]
+    for m, n in shapes:
+        a = rand100((m, n), ao)
+        c = np.zeros((n, m), np.float32, order=co)
+        helpers.oracle_transpose(c, a, block=8)
+        assert np.array_equal(c, a.T), (ao, co, m, n)
+
+
+def test_transpose_in_place_exact():
+    """test/test.trans.cpp:44-74: square in place, sizes 2..31 and multi-block."""
+    for n in list(range(2, 32)) + [65, 100]:
+        a = rand100((n, n), "F")
+        want = a.T.copy()
+        helpers.oracle_transpose_inplace(a, block=8)
+        assert np.array_equal(a, want), n
