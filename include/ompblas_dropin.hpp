@@ -28,8 +28,10 @@
 #ifndef OMPBLAS_DROPIN_HPP
 #define OMPBLAS_DROPIN_HPP
 
+#include <array>
 #include <cstddef>
 #include <cstdint>
+#include <optional>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -271,6 +273,180 @@ void transpose_helper(ValueType* c, SizeType const* nc, SizeType const* wc, Valu
         dropin::check(omb_stranspose_inplace(n, A.get(), ld, o.stream), "amt::transpose_helper");
         A.finish();
     }
+}
+
+// ------------------------------------------------------------------------
+// Tensor-level closures (include/mtm.hpp:208-267, include/mtv.hpp:100-236,
+// include/dot.hpp:176-224, include/outer.hpp:51-122, include/trans.hpp:82-166):
+// validate shapes when the closure is made (std::runtime_error, as the
+// reference) and run the helper when it is called.  Any 2-D tensor type with
+// data(), extents() and strides() (indexable, size() == 2) works, uBLAS
+// tensors included; amt::matrix_view below is a minimal one over caller
+// memory.  Unlike the reference, the closures copy extents and strides, so
+// only the data must outlive them.
+
+// Row/column extents and strides of a caller-owned 2-D array.
+template <class T>
+struct matrix_view {
+    T* ptr;
+    std::array<std::size_t, 2> n;
+    std::array<std::size_t, 2> w;
+    T* data() const { return ptr; }
+    const std::array<std::size_t, 2>& extents() const { return n; }
+    const std::array<std::size_t, 2>& strides() const { return w; }
+    // rows x cols with the unit stride along the rows (first order) or the
+    // columns (last order)
+    static matrix_view first_order(T* p, std::size_t rows, std::size_t cols) {
+        return {p, {rows, cols}, {1, rows}};
+    }
+    static matrix_view last_order(T* p, std::size_t rows, std::size_t cols) {
+        return {p, {rows, cols}, {cols, 1}};
+    }
+};
+
+namespace dropin {
+template <class Tn>
+std::array<std::size_t, 2> ext2(const Tn& t) {
+    const auto& e = t.extents();
+    if (e.size() != 2) throw std::runtime_error("amt: only rank-2 tensors are supported");
+    return {static_cast<std::size_t>(e[0]), static_cast<std::size_t>(e[1])};
+}
+template <class Tn>
+std::array<std::size_t, 2> str2(const Tn& t) {
+    const auto& w = t.strides();
+    return {static_cast<std::size_t>(w[0]), static_cast<std::size_t>(w[1])};
+}
+inline bool is_vec(const std::array<std::size_t, 2>& n) { return n[0] == 1 || n[1] == 1; }
+inline bool first_order_of(const std::array<std::size_t, 2>& n,
+                           const std::array<std::size_t, 2>& w) {
+    return w[0] == 1 && !(w[1] == 1 && n[1] > 1);
+}
+}  // namespace dropin
+
+template <class TC, class TA, class TB>
+auto mtm(TC& c, const TA& a, const TB& b, std::optional<std::size_t> num_threads = std::nullopt) {
+    (void)num_threads;
+    const auto na = dropin::ext2(a), nb = dropin::ext2(b), nc = dropin::ext2(c);
+    if (!(na[0] == nc[0] && na[1] == nb[0] && nc[1] == nb[1]))
+        throw std::runtime_error("amt::mtm: dimension mismatch");
+    const auto wa = dropin::str2(a), wb = dropin::str2(b), wc = dropin::str2(c);
+    auto* cp = c.data();
+    const auto* ap = a.data();
+    const auto* bp = b.data();
+    return [=] {
+        mtm_helper(cp, nc.data(), wc.data(), ap, na.data(), wa.data(), bp, nb.data(), wb.data(),
+                   layout::last_order{});
+    };
+}
+
+// c += A b (first-order A) or c = A b (last-order A).
+template <class TC, class TA, class TB>
+auto mtv(TC& c, const TA& a, const TB& b, std::optional<std::size_t> num_threads = std::nullopt) {
+    (void)num_threads;
+    const auto na = dropin::ext2(a), nb = dropin::ext2(b), nc = dropin::ext2(c);
+    if (!(dropin::is_vec(nb) && dropin::is_vec(nc)))
+        throw std::runtime_error("amt::mtv: c and b must be vector, and a must be a matrix");
+    if (nc[0] * nc[1] != na[0] || nb[0] * nb[1] != na[1])
+        throw std::runtime_error("amt::mtv: dimension mismatch");
+    const auto wa = dropin::str2(a);
+    const bool first = dropin::first_order_of(na, wa);
+    auto* cp = c.data();
+    const auto* ap = a.data();
+    const auto* bp = b.data();
+    return [=] {
+        if (first)
+            mtv_helper(cp, nc.data(), ap, na.data(), wa.data(), bp, nb.data(), 1, layout::first_order{});
+        else
+            mtv_helper(cp, nc.data(), ap, na.data(), wa.data(), bp, nb.data(), 1, layout::last_order{});
+    };
+}
+
+// c = A^T b, through the helper of the other order on the transposed view:
+// first-order A overwrites c, last-order A accumulates into it (as the
+// reference's vtm, but the true transpose for every shape).
+template <class TC, class TA, class TB>
+auto vtm(TC& c, const TA& a, const TB& b, std::optional<std::size_t> num_threads = std::nullopt) {
+    (void)num_threads;
+    const auto na = dropin::ext2(a), nb = dropin::ext2(b), nc = dropin::ext2(c);
+    if (!(dropin::is_vec(nb) && dropin::is_vec(nc)))
+        throw std::runtime_error("amt::vtm: c and b must be vector, and a must be a matrix");
+    if (nc[0] * nc[1] != na[1] || nb[0] * nb[1] != na[0])
+        throw std::runtime_error("amt::vtm: dimension mismatch");
+    const auto wa = dropin::str2(a);
+    const std::array<std::size_t, 2> nt = {na[1], na[0]}, wt = {wa[1], wa[0]};
+    const bool first_t = dropin::first_order_of(nt, wt);
+    auto* cp = c.data();
+    const auto* ap = a.data();
+    const auto* bp = b.data();
+    return [=] {
+        if (first_t)
+            mtv_helper(cp, nc.data(), ap, nt.data(), wt.data(), bp, nb.data(), 1, layout::first_order{});
+        else
+            mtv_helper(cp, nc.data(), ap, nt.data(), wt.data(), bp, nb.data(), 1, layout::last_order{});
+    };
+}
+
+template <class TC, class TA, class TB>
+auto dot_prod(TC& c, const TA& a, const TB& b,
+              std::optional<std::size_t> num_threads = std::nullopt) {
+    (void)num_threads;
+    const auto na = dropin::ext2(a), nb = dropin::ext2(b);
+    if (!(dropin::is_vec(na) && dropin::is_vec(nb)))
+        throw std::runtime_error("amt::dot_prod: both tensor must be vector");
+    if (na[0] * na[1] != nb[0] * nb[1])
+        throw std::runtime_error("amt::dot_prod: both vector must be of same size");
+    auto* cp = c.data();
+    const auto* ap = a.data();
+    const auto* bp = b.data();
+    return [=] { dot_prod_helper(cp, ap, na.data(), bp, nb.data(), 1); };
+}
+
+template <class TC, class TA, class TB>
+auto outer_prod(TC& c, const TA& a, const TB& b,
+                std::optional<std::size_t> num_threads = std::nullopt) {
+    (void)num_threads;
+    const auto na = dropin::ext2(a), nb = dropin::ext2(b), nc = dropin::ext2(c);
+    if (!(dropin::is_vec(na) && dropin::is_vec(nb)))
+        throw std::runtime_error("amt::outer_prod: both tensor must be vector");
+    if (nc[0] != na[0] * na[1] || nc[1] != nb[0] * nb[1])
+        throw std::runtime_error("amt::outer_prod: dimension mismatch");
+    const auto wc0 = dropin::str2(c);
+    const bool first = dropin::first_order_of(nc, wc0);
+    auto* cp = c.data();
+    const auto* ap = a.data();
+    const auto* bp = b.data();
+    // first order: C's columns are the lines; last order: its rows, a and b swapped
+    const std::array<std::size_t, 2> wc = first ? std::array<std::size_t, 2>{1, wc0[1]}
+                                                : std::array<std::size_t, 2>{wc0[0], 1};
+    return [=] {
+        if (first)
+            outer_prod_helper(cp, nc.data(), wc.data(), ap, na.data(), bp, nb.data(), 1);
+        else
+            outer_prod_helper(cp, nc.data(), wc.data(), bp, nb.data(), ap, na.data(), 1);
+    };
+}
+
+template <class TC, class TA>
+auto transpose(TC& c, const TA& a, std::optional<std::size_t> num_threads = std::nullopt) {
+    (void)num_threads;
+    const auto na = dropin::ext2(a), nc = dropin::ext2(c);
+    if (!(na[0] == nc[1] && na[1] == nc[0]))
+        throw std::runtime_error("amt::transpose: dimension mismatch");
+    const auto wa = dropin::str2(a), wc = dropin::str2(c);
+    auto* cp = c.data();
+    const auto* ap = a.data();
+    return [=] { transpose_helper(cp, nc.data(), wc.data(), ap, na.data(), wa.data(), tag::outplace{}); };
+}
+
+template <class TA>
+auto transpose(TA& a, std::optional<std::size_t> num_threads = std::nullopt) {
+    (void)num_threads;
+    const auto na = dropin::ext2(a);
+    if (na[0] != na[1])
+        throw std::runtime_error("amt::transpose: in-place transpose needs a square matrix");
+    const auto wa = dropin::str2(a);
+    auto* ap = a.data();
+    return [=] { transpose_helper(ap, na.data(), wa.data(), ap, na.data(), wa.data(), tag::inplace{}); };
 }
 
 }  // namespace amt

@@ -322,6 +322,101 @@ void test_outer_transpose(bool quick) {
     transpose_inplace_case(300, false);
 }
 
+// Tensor-level closures over amt::matrix_view (host memory: the closures run
+// the helpers, which stage it): shapes the reference's closure tests use,
+// rectangular ones, and the shape errors raised when a closure is made.
+void test_tensor_level() {
+    using V = amt::matrix_view<float>;
+    using CV = amt::matrix_view<const float>;
+    for (std::size_t m : {3, 17, 40}) {
+        const std::size_t k = m + 5, n = 2 * m + 1;
+        Mat A(m, k, true), B(k, n, false), C(m, n, false);
+        rand_fill(A.v);
+        rand_fill(B.v);
+        std::vector<int64_t> want(m * n, 0);
+        for (std::size_t i = 0; i < m; ++i)
+            for (std::size_t j = 0; j < n; ++j)
+                for (std::size_t q = 0; q < k; ++q)
+                    want[i * n + j] += static_cast<int64_t>(A.at(i, q)) * static_cast<int64_t>(B.at(q, j));
+        auto cv = V::last_order(C.v.data(), m, n);
+        amt::mtm(cv, CV::first_order(A.v.data(), m, k), CV::last_order(B.v.data(), k, n))();
+        for (std::size_t i = 0; i < m; ++i)
+            for (std::size_t j = 0; j < n; ++j)
+                CHECK(static_cast<int64_t>(C.at(i, j)) == want[i * n + j], "tensor mtm %zu (%zu,%zu)", m, i, j);
+        // mtv (first order accumulates), vtm (true transpose)
+        std::vector<float> x(k), y(m, 0.f), z(k, 0.f), u(m);
+        rand_fill(x);
+        rand_fill(u);
+        auto yv = V::last_order(y.data(), 1, m);
+        amt::mtv(yv, CV::first_order(A.v.data(), m, k), CV::last_order(x.data(), 1, k))();
+        auto zv = V::last_order(z.data(), 1, k);
+        amt::vtm(zv, CV::first_order(A.v.data(), m, k), CV::last_order(u.data(), 1, m))();
+        for (std::size_t i = 0; i < m; ++i) {
+            int64_t s = 0;
+            for (std::size_t q = 0; q < k; ++q) s += static_cast<int64_t>(A.at(i, q)) * static_cast<int64_t>(x[q]);
+            CHECK(static_cast<int64_t>(y[i]) == s, "tensor mtv %zu row %zu", m, i);
+        }
+        for (std::size_t q = 0; q < k; ++q) {
+            int64_t s = 0;
+            for (std::size_t i = 0; i < m; ++i) s += static_cast<int64_t>(A.at(i, q)) * static_cast<int64_t>(u[i]);
+            CHECK(static_cast<int64_t>(z[q]) == s, "tensor vtm %zu col %zu", m, q);
+        }
+        float r = 0.f;
+        amt::matrix_view<float> rv{&r, {1, 1}, {1, 1}};
+        amt::dot_prod(rv, CV::last_order(x.data(), 1, k), CV::last_order(x.data(), 1, k))();
+        int64_t dd = 0;
+        for (float t : x) dd += static_cast<int64_t>(t) * static_cast<int64_t>(t);
+        CHECK(static_cast<int64_t>(r) == dd, "tensor dot %zu", m);
+        // outer product into both orders, transpose out of and in place
+        for (int first = 0; first < 2; ++first) {
+            Mat O(m, k, first);
+            std::fill(O.v.begin(), O.v.end(), 1.f);
+            auto ov = first ? V::first_order(O.v.data(), m, k) : V::last_order(O.v.data(), m, k);
+            amt::outer_prod(ov, CV::last_order(u.data(), 1, m), CV::last_order(x.data(), 1, k))();
+            for (std::size_t i = 0; i < m; ++i)
+                for (std::size_t q = 0; q < k; ++q)
+                    CHECK(O.at(i, q) == 1.f + u[i] * x[q], "tensor outer %d %zu (%zu,%zu)", first, m, i, q);
+        }
+        Mat T(k, m, false);
+        auto tv = V::last_order(T.v.data(), k, m);
+        amt::transpose(tv, CV::first_order(A.v.data(), m, k))();
+        for (std::size_t i = 0; i < m; ++i)
+            for (std::size_t q = 0; q < k; ++q) CHECK(T.at(q, i) == A.at(i, q), "tensor transpose %zu", m);
+        Mat S(m, m, true);
+        rand_fill(S.v);
+        const std::vector<float> s0 = S.v;
+        auto sv = V::first_order(S.v.data(), m, m);
+        amt::transpose(sv)();
+        for (std::size_t i = 0; i < m; ++i)
+            for (std::size_t j = 0; j < m; ++j) CHECK(S.v[i + j * m] == s0[j + i * m], "tensor transpose in place %zu", m);
+    }
+    // shape errors at closure creation
+    std::vector<float> buf(64);
+    auto expect_throw = [&](auto&& make, const char* what) {
+        bool threw = false;
+        try {
+            make();
+        } catch (const std::runtime_error& e) {
+            threw = std::strstr(e.what(), what) != nullptr;
+        }
+        CHECK(threw, "expected '%s'", what);
+    };
+    auto m34 = V::last_order(buf.data(), 3, 4);
+    auto m52 = CV::last_order(buf.data(), 5, 2);
+    auto m32 = V::last_order(buf.data(), 3, 2);
+    expect_throw([&] { amt::mtm(m32, CV::last_order(buf.data(), 3, 4), m52); }, "dimension mismatch");
+    auto v3 = V::last_order(buf.data(), 1, 3);
+    expect_throw([&] { amt::mtv(v3, CV::last_order(buf.data(), 3, 4), CV::last_order(buf.data(), 1, 5)); },
+                 "dimension mismatch");
+    expect_throw([&] { amt::mtv(m34, CV::last_order(buf.data(), 3, 4), CV::last_order(buf.data(), 1, 4)); },
+                 "must be vector");
+    expect_throw([&] { amt::dot_prod(v3, CV::last_order(buf.data(), 1, 4), CV::last_order(buf.data(), 1, 5)); },
+                 "same size");
+    expect_throw([&] { amt::outer_prod(m34, CV::last_order(buf.data(), 1, 3), CV::last_order(buf.data(), 1, 5)); },
+                 "dimension mismatch");
+    expect_throw([&] { amt::transpose(m34); }, "square");
+}
+
 void test_errors() {
     // only float is computed
     double c = 0, a[4] = {1, 2, 3, 4};
@@ -367,6 +462,8 @@ int main(int argc, char** argv) {
         }
         test_outer_transpose(quick);
         std::printf("outer/transpose: %d checks, %d failures\n", g_checks, g_failures);
+        test_tensor_level();
+        std::printf("tensor-level closures: %d checks, %d failures\n", g_checks, g_failures);
         test_errors();
         amt::dropin::settings().gemm_mode = OMB_MODE_SPLIT_BF16;
         for (std::size_t sz : {5, 31}) mtm_case<amt::layout::last_order>(false, true, false, sz, false, "LFL split");
