@@ -6,7 +6,8 @@
 // micro-kernel.  Here a 256-thread workgroup owns a 128x128 tile of C, stages
 // 128x16 slices of A and B through LDS (double buffered; the staging plays
 // the role of `pack`, including the layout change for either operand order),
-// and every thread keeps an 8x8 register micro-tile.
+// and every thread keeps an 8x8 register micro-tile.  Global loads run two
+// slices ahead in registers and LDS fragments one k ahead.
 //
 // Accumulation order: each thread sums 128 consecutive k (8 LDS slices) into
 // an inner register tile, then adds it to an outer tile.  This two-level sum
@@ -135,18 +136,10 @@ __global__ __launch_bounds__(NT, 2) void sgemm_f32_kernel(Args p) {
         for (int j = 0; j < 8; ++j) inner[i][j] = outer[i][j] = 0.f;
 
     const int64_t nk = ceil_div(p.K, BK);
-    float4 ra[2], rb[2];
-    load_slice<A_KC, B_NC, VEC>(p, m0, n0, 0, tid, ra, rb);
-    store_slice<A_KC, B_NC>(As[0], Bs[0], tid, ra, rb);
-    __syncthreads();
-
     constexpr int kFold = INNER_K / BK;
-    for (int64_t kt = 0; kt < nk; ++kt) {
-        const int cur = kt & 1;
-        const bool more = kt + 1 < nk;
-        if (more) load_slice<A_KC, B_NC, VEC>(p, m0, n0, (kt + 1) * BK, tid, ra, rb);
-        // LDS fragments in two register sets: row k+1 is read while row k's
-        // 64 FMAs run (one set only left the LDS latency exposed every k).
+    // Slices kt and kt+1 live in LDS buffers 0 and 1; global loads run two
+    // slices ahead in two register sets.
+    auto compute = [&](int cur, int64_t kt) {
         float4 fa[2][2], fb[2][2];
         auto fragment = [&](int k, int set) {
             fa[set][0] = *reinterpret_cast<const float4*>(&As[cur][k][ty * 4]);
@@ -167,7 +160,7 @@ __global__ __launch_bounds__(NT, 2) void sgemm_f32_kernel(Args p) {
 #pragma unroll
                 for (int j = 0; j < 8; ++j) inner[i][j] = fmaf(a[i], b[j], inner[i][j]);
         }
-        if ((kt + 1) % kFold == 0 || !more) {
+        if ((kt + 1) % kFold == 0 || kt + 1 == nk) {
 #pragma unroll
             for (int i = 0; i < 8; ++i)
 #pragma unroll
@@ -176,7 +169,21 @@ __global__ __launch_bounds__(NT, 2) void sgemm_f32_kernel(Args p) {
                     inner[i][j] = 0.f;
                 }
         }
-        if (more) store_slice<A_KC, B_NC>(As[cur ^ 1], Bs[cur ^ 1], tid, ra, rb);
+    };
+    float4 ra0[2], rb0[2], ra1[2], rb1[2];
+    load_slice<A_KC, B_NC, VEC>(p, m0, n0, 0, tid, ra0, rb0);
+    store_slice<A_KC, B_NC>(As[0], Bs[0], tid, ra0, rb0);
+    if (nk > 1) load_slice<A_KC, B_NC, VEC>(p, m0, n0, BK, tid, ra1, rb1);
+    __syncthreads();
+    for (int64_t kt = 0; kt < nk; kt += 2) {
+        if (kt + 2 < nk) load_slice<A_KC, B_NC, VEC>(p, m0, n0, (kt + 2) * BK, tid, ra0, rb0);
+        compute(0, kt);
+        if (kt + 1 < nk) store_slice<A_KC, B_NC>(As[1], Bs[1], tid, ra1, rb1);
+        __syncthreads();
+        if (kt + 1 >= nk) break;
+        if (kt + 3 < nk) load_slice<A_KC, B_NC, VEC>(p, m0, n0, (kt + 3) * BK, tid, ra1, rb1);
+        compute(1, kt + 1);
+        if (kt + 2 < nk) store_slice<A_KC, B_NC>(As[0], Bs[0], tid, ra0, rb0);
         __syncthreads();
     }
 
