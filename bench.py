@@ -549,7 +549,7 @@ def serving_leg(pkg, torch, dev, stream, rate, count, slots):
                               if (kinds == k).any()},
            "service_p99_ms": float(np.percentile(svc, 99)) * 1e3}
     sweep = []
-    for r in (8000.0, 16000.0, 48000.0):
+    for r in (8000.0, 16000.0, 48000.0, 64000.0, 96000.0):
         tr = pkg.serving_trace(count, r, seed=4)
         reqs = requests()
         for q, t in zip(reqs, tr):

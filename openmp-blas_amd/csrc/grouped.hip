@@ -391,8 +391,8 @@ Key key_of(int mode, const omb_gemm_desc& d) { return Key{mode, lg(d.M), lg(d.N)
 // tiles when the smaller output dimension is under 48, else 64x64.  (The
 // 128x64 class needs the larger-register kernel, 2 workgroups per CU instead
 // of 3, which slows every other item of a mixed plan; at C4's shapes 64x64
-// gives the same total rate, so 128x64 is used only when tuning shows a
-// clear win.)
+// gives the same total rate, so 128x64 is used only when a loaded tuning
+// table names it.)
 int default_class(int mode, const omb_gemm_desc& d) {
     if (mode != OMB_MODE_PLAIN_F32) return CSPLIT;
     return std::min(d.M, d.N) < 48 ? C32 : C64;
@@ -726,9 +726,12 @@ extern "C" omb_status omb_autotune_group(const omb_gemm_desc* gemms, int32_t n_g
     for (int i = 0; i < n_gemm; ++i)
         if (gemms[i].M > 0 && gemms[i].N > 0) buckets[key_of(mode, gemms[i])].push_back(gemms[i]);
     const int batch = 64;
-    const std::vector<int> cands = mode == OMB_MODE_PLAIN_F32
-                                       ? std::vector<int>{C32, C64, C128x64}
-                                       : std::vector<int>{CSPLIT};
+    // 128x64 is not a candidate: timed alone it can win a bucket, but it
+    // needs the larger-register kernel (2 workgroups per CU instead of 3),
+    // which slows every other item of the plan (see default_class); a
+    // table loaded with omb_autotune_load may still name it.
+    const std::vector<int> cands = mode == OMB_MODE_PLAIN_F32 ? std::vector<int>{C32, C64}
+                                                              : std::vector<int>{CSPLIT};
     int64_t maxM = 1, maxN = 1, maxK = 1;
     for (auto& kv : buckets)
         for (auto& d : kv.second) {
@@ -768,9 +771,6 @@ extern "C" omb_status omb_autotune_group(const omb_gemm_desc* gemms, int32_t n_g
             float ms = 1e30f;
             st = time_class(mode, rep.M, rep.N, rep.K, c, batch, A, B, C, s, &ms);
             if (st != OMB_OK) break;
-            // 128x64 must beat the others by 20%: it costs the rest of a
-            // mixed plan one workgroup per CU (see default_class)
-            if (c == C128x64) ms *= 1.2f;
             if (ms < best_ms) {
                 best_ms = ms;
                 best = c;

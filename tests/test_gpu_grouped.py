@@ -153,3 +153,31 @@ def test_grouped_integer_exact(dev, ompblas):
         y = b.vy[b.ovy[i]:b.ovy[i] + m].double()
         assert torch.equal(y, a @ x), i
     plan.close()
+
+
+def test_loaded_table_can_select_the_128x64_class(dev, ompblas, tmp_path):
+    """A loaded tuning table may name f32_128x64 (the autotuner no longer
+    proposes it): that plan runs the larger-register kernel and stays bitwise
+    equal to per-instance calls.  Buckets with M, N >= 64 are then mapped back
+    to f32_64x64, which is their default class, so later tests see no change."""
+    gs, _ = shapes(13, 80, 0, lo=64, hi=300)
+    b = Batch(dev, ompblas, gs, [], seed=13)
+    want_c, _ = b.eager(ompblas, 0)
+    lg = lambda v: int(v).bit_length() - 1
+    keys = sorted({(lg(m), lg(n), lg(k)) for m, n, k in gs})
+    big = tmp_path / "big.txt"
+    big.write_text("".join(f"0 {bm} {bn} {bk} f32_128x64\n" for bm, bn, bk in keys))
+    back = tmp_path / "back.txt"
+    back.write_text("".join(f"0 {bm} {bn} {bk} f32_64x64\n" for bm, bn, bk in keys))
+    ompblas.autotune_load(str(big))
+    try:
+        b.reset()
+        plan = ompblas.GroupPlan(b.gemm_descs(), [], 0)
+        desc = plan.describe()
+        assert "kernel=big" in desc and "f32_128x64" in desc, desc
+        plan.launch()
+        torch.cuda.synchronize()
+        assert torch.equal(b.gc, want_c)
+        plan.close()
+    finally:
+        ompblas.autotune_load(str(back))
