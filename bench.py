@@ -212,6 +212,23 @@ def main():
                          "frac_of_fp32_peak": flop / t_p / PEAK_FP32}
     if plain:
         kernels["sgemm_f32_kernel (8192^3 row-major)"] = dict(plain["row"])
+        # The FP32 FMA rate the CUs sustain with nothing but independent
+        # v_fmac_f32 chains, at the plain kernel's occupancy (2 workgroups of
+        # 256 threads per CU) and at 4: the practical ceiling of the VALU path.
+        with torch.cuda.stream(stream):
+            sink = torch.zeros(1, device=dev)
+        cus = torch.cuda.get_device_properties(dev).multi_processor_count
+        ceiling = {}
+        for per_cu in (2, 4):
+            blocks, iters = per_cu * cus, 8192
+            fn = lambda: lib.omb_calib_fmac(blocks, iters, sink.data_ptr(), sp)
+            pkg.check(fn())
+            t_c = timer.run(fn, 5)
+            ceiling[f"{per_cu}_workgroups_per_cu"] = blocks * 256 * iters * 128 / t_c / 1e12
+        best = max(ceiling.values())
+        kernels["valu_fmac_ceiling"] = {
+            "tflops": ceiling, "frac_of_fp32_peak": best / (PEAK_FP32 / 1e12),
+            "plain_row_frac_of_ceiling": plain["row"]["tflops"] / best}
 
     # ------------------------------------------------------------- errors --
     errors = {}

@@ -133,6 +133,12 @@ omb_status omb_check_ddot(int64_t n, const float* x, const float* y, double* r64
  * out[3] = sum C64^2 (device array of 4 doubles).  Deterministic. */
 omb_status omb_check_errnorm(int64_t M, int64_t N, const float* C, int64_t rsC, int64_t csC,
                              const double* C64, double* out, omb_stream_t stream);
+/* VALU issue ceiling (measurement only, no counterpart in the reference):
+ * `blocks` workgroups of 256 threads each run `iters` rounds of 64
+ * independent FP32 multiply-adds per thread (blocks * 256 * iters * 128 FLOP)
+ * in the instruction form of the plain SGEMM kernel.  Timing it gives the
+ * FP32 FMA rate the CUs sustain at that occupancy.  sink: 4 device bytes. */
+omb_status omb_calib_fmac(int64_t blocks, int64_t iters, float* sink, omb_stream_t stream);
 
 /* ----------------------------------------------------- grouped launch ----
  * Thousands of independent GEMMs (C_i = alpha*A_i*B_i + beta*C_i, every

@@ -160,6 +160,29 @@ unsigned grid_for(int64_t n, int64_t cap) {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, NT), cap));
 }
 
+// VALU issue ceiling: 64 independent FP32 multiply-add chains per thread, in
+// the two-VGPR-operand form (v_fmac_f32) that the plain SGEMM kernel's inner
+// loop consists of.  Nothing but the chains runs in the loop, so the rate it
+// reaches is the most FP32 FMA work the CUs retire at the occupancy of the
+// launch.  The single store is taken only when the sum is exactly 0.125 (the
+// inputs make that unreachable) and goes to sink[0].
+__global__ __launch_bounds__(NT) void fmac_ceiling_kernel(int64_t iters, float* sink) {
+    constexpr int CH = 64;
+    float acc[CH];
+    const float x0 = 1.0f + threadIdx.x * 0x1p-20f, x1 = x0 + 0x1p-12f;
+    const float y0 = 0.5f - threadIdx.x * 0x1p-21f, y1 = y0 - 0x1p-13f;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) acc[c] = threadIdx.x + c;
+    for (int64_t it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int c = 0; c < CH; ++c) acc[c] = fmaf((c & 1) ? x1 : x0, (c & 2) ? y1 : y0, acc[c]);
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) s += acc[c];
+    if (s == 0.125f) sink[0] = s;
+}
+
 }  // namespace
 
 omb_status launch_fill_uniform(float* dst, int64_t n, uint64_t seed, uint64_t stream_id,
@@ -249,6 +272,19 @@ extern "C" omb_status omb_check_errnorm(int64_t M, int64_t N, const float* C, in
     hipLaunchKernelGGL(errnorm_partial_kernel, dim3(blocks), dim3(NT), 0, s, M, N, C, rsC, csC,
                        C64, part);
     hipLaunchKernelGGL(combine_kernel, dim3(1), dim3(1), 0, s, part, (int64_t)blocks, 4, out);
+    OMB_HIP_TRY(hipGetLastError());
+    return OMB_OK;
+}
+
+extern "C" omb_status omb_calib_fmac(int64_t blocks, int64_t iters, float* sink,
+                                     omb_stream_t stream) {
+    clear_error();
+    if (blocks < 1 || blocks > 0x7fffffff || iters < 1 || !sink) {
+        set_error("omb_calib_fmac: invalid argument");
+        return OMB_EINVAL;
+    }
+    hipLaunchKernelGGL(fmac_ceiling_kernel, dim3((unsigned)blocks), dim3(NT), 0,
+                       (hipStream_t)stream, iters, sink);
     OMB_HIP_TRY(hipGetLastError());
     return OMB_OK;
 }

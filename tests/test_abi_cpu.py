@@ -68,6 +68,10 @@ def test_check_and_fill_reject_bad_arguments(ompblas):
     assert lib.omb_fill_uniform(None, -1, 0, 0, 0, None) == -1
     assert lib.omb_check_dgemm(-1, 1, 1, 16, 1, 1, 16, 1, 1, None, 1, 1, 16, None) == -1
     assert lib.omb_check_errnorm(2, 2, 16, 2, 1, 16, None, None) == -1
+    assert lib.omb_calib_fmac(0, 10, 16, None) == -1
+    assert lib.omb_calib_fmac(4, 0, 16, None) == -1
+    assert lib.omb_calib_fmac(4, 10, None, None) == -1
+    assert "omb_calib_fmac" in ompblas.last_error()
 
 
 def test_torch_mirror_shape_errors(ompblas):

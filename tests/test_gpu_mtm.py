@@ -104,6 +104,59 @@ def test_mtm_alpha_beta_and_k_zero(dev, ompblas):
     assert torch.equal(c, 3.0 * c0)
 
 
+@pytest.mark.parametrize("orders", ["CC", "FF", "CF"])
+def test_nonfinite_inputs(dev, ompblas, orders):
+    """Plain FP32 follows IEEE exactly as the reference's FP32 loops do: an Inf
+    or NaN input makes the outputs it feeds Inf or NaN, and nothing else.  The
+    split modes turn every output an Inf or NaN input feeds into NaN (the
+    split of an infinity is not finite).  Outputs no special value feeds are
+    unchanged (integer inputs: exact)."""
+    M, K, N = 40, 30, 35
+    a = rand100((M, K), orders[0])
+    b = rand100((K, N), orders[1])
+    a[3, 7], a[10, 2], a[20, 5] = np.inf, np.nan, -np.inf
+    b[4, 9] = np.inf
+    # IEEE products in FP64, summed per element (Inf*0 and Inf-Inf give NaN)
+    truth = (a.astype(np.float64)[:, :, None] * b.astype(np.float64)[None, :, :]).sum(axis=1)
+    ta, tb = to_dev(a, dev), to_dev(b, dev)
+    fed_rows, fed_col = [3, 10, 20], 9
+    for mode in MODES:
+        tc = torch.zeros(M, N, device=dev)
+        run_mtm(ompblas, tc, ta, tb, mode)
+        got = tc.cpu().numpy()
+        if mode == 0:
+            assert np.array_equal(got, truth.astype(np.float32), equal_nan=True)
+            continue
+        assert np.isnan(got[fed_rows, :]).all() and np.isnan(got[:, fed_col]).all(), mode
+        keep = np.ones((M, N), bool)
+        keep[fed_rows, :] = False
+        keep[:, fed_col] = False
+        assert np.array_equal(got[keep], truth.astype(np.float32)[keep]), mode
+
+
+@pytest.mark.parametrize("orders", ["CC", "FF"])
+def test_split_values_beyond_bf16_range_stay_finite(dev, ompblas, orders):
+    """Finite |x| above BF16's largest finite value (about 3.3895e38; values
+    above 3.3962e38 round to infinity in BF16) still split exactly: FLT_MAX-
+    sized entries of A against tiny entries of B give finite C within FP32
+    rounding of the FP64 product, in every mode."""
+    M, K, N = 6, 6, 5
+    a = rand100((M, K), orders[0])
+    b = rand100((K, N), orders[1])
+    fmax = np.finfo(np.float32).max
+    a[:, 0] = np.array([fmax, -fmax, 3.3965e38, -3.397e38, 3.0e38, 1.0], np.float32)
+    b[0, :] = b[0, :] * np.float32(2.0 ** -110)
+    truth = a.astype(np.float64) @ b.astype(np.float64)
+    ta, tb = to_dev(a, dev), to_dev(b, dev)
+    for mode in MODES:
+        tc = torch.zeros(M, N, device=dev)
+        run_mtm(ompblas, tc, ta, tb, mode)
+        got = tc.cpu().numpy().astype(np.float64)
+        assert np.isfinite(got).all(), mode
+        tol = 1e-6 if mode < 2 else 2e-5
+        assert np.all(np.abs(got - truth) <= tol * np.abs(truth) + 1e-3), mode
+
+
 def _fp_case(dev, ompblas, M, N, K, seed, c_order="C", a_order="C", b_order="C"):
     a = helpers.rng_uniform(M * K, seed, 0).reshape((M, K), order=a_order)
     b = helpers.rng_uniform(K * N, seed, 1).reshape((K, N), order=b_order)
