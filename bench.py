@@ -83,7 +83,7 @@ def main():
                     help="seconds of host time for the CPU baseline sample")
     ap.add_argument("--legs", default="all",
                     help="comma list of legs beside the headline: plain, errors, gemv, dot, "
-                         "level2, grouped, serving, cpu (default all)")
+                         "level2, grouped, serving, c1, cpu (default all)")
     ap.add_argument("--serve-rate", type=float, default=float(os.environ.get("OMB_SERVE_RATE", 32000)),
                     help="serving leg: Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-count", type=int, default=int(os.environ.get("OMB_SERVE_COUNT", 3000)),
@@ -96,7 +96,7 @@ def main():
               "(the multi-GPU sharding is a design, not a measured path)", file=sys.stderr)
         sys.exit(2)
 
-    legs = {"plain", "errors", "gemv", "dot", "level2", "grouped", "serving", "cpu"}
+    legs = {"plain", "errors", "gemv", "dot", "level2", "grouped", "serving", "c1", "cpu"}
     if args.legs != "all":
         legs = {x.strip() for x in args.legs.split(",") if x.strip()}
     if args.no_errors:
@@ -233,7 +233,7 @@ def main():
     # ------------------------------------------------------------- errors --
     errors = {}
     if "errors" in legs:
-        errors = error_leg(pkg, lib, torch, dev, sp, A, B, C)
+        errors = error_leg(pkg, lib, torch, dev, stream, A, B, C)
     sync()
 
     # --------------------------------------------------------- GEMV / dot --
@@ -319,17 +319,23 @@ def main():
     # ------------------------------------------------ grouped and serving --
     grouped, serving = None, None
     if "grouped" in legs:
-        grouped = grouped_leg(pkg, lib, torch, dev, stream, timer, args.steps)
+        grouped = grouped_leg(pkg, lib, torch, dev, stream, timer, args.steps, "cpu" in legs)
         sync()
     if "serving" in legs:
         serving = serving_leg(pkg, torch, dev, stream, args.serve_rate, args.serve_count,
                               args.serve_slots)
         sync()
 
+    c1 = None
+    if "c1" in legs:
+        c1 = c1_leg(pkg, lib, torch, dev, sp, timer)
+        sync()
+
     # ------------------------------------------------------- CPU baseline --
-    cpu = None
+    cpu, cpu_ref = None, None
     if "cpu" in legs:
         cpu = cpu_baseline(args.cpu_budget)
+        cpu_ref = cpu_reference_legs(pkg, torch, dev, stream)
 
     res = {
         "metric": METRIC,
@@ -356,6 +362,8 @@ def main():
                      "algorithmic_bytes": 2 * 3 * n * n * 2 + 4 * n * n,
                      "work_per_launch": "6 products x 2*8192^3 = 6.597e12 MFMA flop"},
         "cpu_baseline": cpu,
+        "cpu_reference_path": cpu_ref,
+        "config1_1024": c1,
         "wall_s_timed_region": round(wall, 4),
         "split_bf16_p3_step_ms": round(t_step3 * 1e3, 4),
         "plain_f32": plain,
@@ -373,12 +381,14 @@ def main():
     print(json.dumps(res), flush=True)
 
 
-def error_leg(pkg, lib, torch, dev, sp, A, B, C):
+def error_leg(pkg, lib, torch, dev, stream, A, B, C):
     """Max- and Frobenius-relative error vs an FP64 product (GPU check kernel)
     of plain FP32 and split-BF16 (6 and 3 products) at 8192^3, for U[-1,1)
-    and sign*2^u (u ~ U[-8,8]) inputs from seed 2.  The reference algorithm's
-    (oracle's) error is measured on a 64-row slab, whose values do not depend
-    on the other rows, and compared with the GPU's on the same slab."""
+    and sign*2^u (u ~ U[-8,8]) inputs from seed 2, and the componentwise
+    error max |C - C64| / (|A| |B|).  The reference algorithm's (oracle's)
+    error is measured on a 64-row slab, whose values do not depend on the
+    other rows, and compared with the GPU's on the same slab."""
+    sp = stream.cuda_stream
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import helpers
 
@@ -386,6 +396,7 @@ def error_leg(pkg, lib, torch, dev, sp, A, B, C):
     rows = slice(4096, 4160)
     out = {}
     C64 = torch.empty((n, n), dtype=torch.float64, device=dev)
+    D64 = torch.empty((n, n), dtype=torch.float64, device=dev)
     for dist in ("uniform", "pow2"):
         log(f"errors vs FP64: {dist}")
         if dist == "uniform":
@@ -399,14 +410,21 @@ def error_leg(pkg, lib, torch, dev, sp, A, B, C):
             A.copy_(torch.from_numpy(a_h))
             B.copy_(torch.from_numpy(b_h))
             torch.cuda.synchronize()
+        # |A| |B| in FP64 for the componentwise error max |C - C64| / (|A| |B|)
+        with torch.cuda.stream(stream):
+            Aabs, Babs = A.abs(), B.abs()
+        pkg.check(lib.omb_check_dgemm(n, n, n, Aabs.data_ptr(), n, 1, Babs.data_ptr(), n, 1, None,
+                                      n, 1, D64.data_ptr(), sp))
         pkg.check(lib.omb_check_dgemm(n, n, n, A.data_ptr(), n, 1, B.data_ptr(), n, 1, None, n, 1,
                                       C64.data_ptr(), sp))
         torch.cuda.synchronize()
+        del Aabs, Babs
         truth = C64[rows].cpu().numpy()
         c_ref = np.zeros((64, n), np.float32)
         helpers.oracle_mtm(c_ref, np.ascontiguousarray(a_h[rows]), b_h)
         ref = helpers.rel_errors(c_ref, truth)
-        d = {"oracle_slab": {"max_rel": ref[0], "frob_rel": ref[1]}}
+        comp_ref = float(np.max(np.abs(c_ref.astype(np.float64) - truth) / D64[rows].cpu().numpy()))
+        d = {"oracle_slab": {"max_rel": ref[0], "frob_rel": ref[1], "componentwise": comp_ref}}
         for mode, name in ((pkg.PLAIN_F32, "plain_f32"), (pkg.SPLIT_BF16, "split_bf16_p6"),
                            (pkg.SPLIT_BF16_P3, "split_bf16_p3")):
             pkg.check(lib.omb_sgemm(n, n, n, 1.0, A.data_ptr(), n, 1, B.data_ptr(), n, 1, 0.0,
@@ -414,15 +432,19 @@ def error_leg(pkg, lib, torch, dev, sp, A, B, C):
             torch.cuda.synchronize()
             full = pkg.rel_errors(C, C64, stream=sp)
             slab = helpers.rel_errors(C[rows].cpu().numpy(), truth)
-            d[name] = {"max_rel": full[0], "frob_rel": full[1],
+            with torch.cuda.stream(stream):
+                comp = float(((C.double() - C64).abs_().div_(D64)).max())
+            comp_slab = float(((C[rows].double() - C64[rows]).abs_().div_(D64[rows])).max())
+            d[name] = {"max_rel": full[0], "frob_rel": full[1], "componentwise": comp,
                        "slab_max_rel": slab[0], "slab_frob_rel": slab[1],
-                       "slab_ratio_vs_oracle": [slab[0] / ref[0], slab[1] / ref[1]]}
+                       "slab_ratio_vs_oracle": [slab[0] / ref[0], slab[1] / ref[1]],
+                       "slab_componentwise_ratio_vs_oracle": comp_slab / comp_ref}
         p = d["plain_f32"]
         for name in ("split_bf16_p6", "split_bf16_p3"):
             d[name]["ratio_vs_plain_f32"] = [d[name]["max_rel"] / p["max_rel"],
                                              d[name]["frob_rel"] / p["frob_rel"]]
         out[dist] = d
-    del C64
+    del C64, D64
     return out
 
 
@@ -436,7 +458,7 @@ def _pool(torch, dev, stream, sizes):
     return buf, offs
 
 
-def grouped_leg(pkg, lib, torch, dev, stream, timer, steps):
+def grouped_leg(pkg, lib, torch, dev, stream, timer, steps, cpu_sample=True):
     """4096 row-major GEMMs (M, N, K log-uniform on [16, 512]) and 4096 GEMVs
     (rows, cols log-uniform on [64, 4096]) from seed 3, operands U[-1,1),
     C = A*B and y = A*x, in one persistent grouped launch replayed from a
@@ -517,8 +539,50 @@ def grouped_leg(pkg, lib, torch, dev, stream, timer, steps):
     per_instance()
     out["per_instance_calls_ms"] = timer.run(per_instance, 2) * 1e3
     out["speedup_vs_per_instance_calls"] = out["per_instance_calls_ms"] / out["plain_f32"]["ms"]
+    if cpu_sample:
+        out["cpu_sequential_sample"] = _grouped_cpu_sample(torch, pools, M, N, K, VM, VN)
     del pools
     return out
+
+
+def _grouped_cpu_sample(torch, pools, M, N, K, VM, VN, count=256):
+    """The reference has no batched API: its path for the grouped batch is one
+    call per instance.  The oracle (the reference algorithm on the host
+    cores) runs the first `count` GEMMs and GEMVs of the batch in sequence,
+    on the same inputs (copied from the device pools); the full-batch time
+    is that sample scaled by FLOP (GEMMs) and bytes (GEMVs)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import helpers
+
+    log(f"grouped: CPU oracle on the first {count} GEMMs and {count} GEMVs")
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    blk = helpers.host_blocking()
+    gi = range(min(count, len(M)))
+    vi = range(min(count, len(VM)))
+    # instances are laid out in order, so the sample is a prefix of each pool
+    ends = [len(pools[j][0]) if (len(gi) if j < 3 else len(vi)) >= len(pools[j][1])
+            else int(pools[j][1][len(gi) if j < 3 else len(vi)]) for j in range(6)]
+    host = [pools[j][0][:ends[j]].cpu().numpy() for j in range(6)]
+    view = lambda j, i, shape: host[j][int(pools[j][1][i]):int(pools[j][1][i]) + int(np.prod(shape))].reshape(shape)
+    cs = [np.zeros((int(M[i]), int(N[i])), np.float32) for i in gi]
+    ys = [np.zeros(int(VM[i]), np.float32) for i in vi]
+    t0 = time.perf_counter()
+    for i in gi:
+        helpers.oracle_mtm(cs[i], view(0, i, (int(M[i]), int(K[i]))), view(1, i, (int(K[i]), int(N[i]))),
+                           blk, cores)
+    t_g = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for i in vi:
+        helpers.oracle_mtv(ys[i], view(3, i, (int(VM[i]), int(VN[i]))), view(4, i, (int(VN[i]),)),
+                           blk, cores)
+    t_v = time.perf_counter() - t0
+    fl = 2.0 * M * N * K
+    vb = 4.0 * (VM * VN + VM + VN)
+    full_ms = (t_g * fl.sum() / fl[:len(gi)].sum() + t_v * vb.sum() / vb[:len(vi)].sum()) * 1e3
+    return {"cores": cores, "gemms": len(gi), "gemvs": len(vi),
+            "gemm_ms": t_g * 1e3, "gemm_tflops": float(fl[:len(gi)].sum()) / t_g / 1e12,
+            "gemv_ms": t_v * 1e3, "gemv_gbps": float(vb[:len(vi)].sum()) / t_v / 1e9,
+            "full_batch_estimate_ms": full_ms}
 
 
 def serving_leg(pkg, torch, dev, stream, rate, count, slots):
@@ -579,6 +643,106 @@ def serving_leg(pkg, torch, dev, stream, rate, count, slots):
     out["fifo_one_slot"] = {"p50_ms": st1["p50_s"] * 1e3, "p99_ms": st1["p99_s"] * 1e3,
                             "aggregate_tflops": st1["total_flop"] / st1["wall_s"] / 1e12}
     del pools
+    return out
+
+
+def _mean_time(fn, iters):
+    """Mean wall time of `iters` calls after one untimed call (the reference's
+    amt::benchmark times MaxIter calls, include/benchmark.hpp:34-51)."""
+    fn()
+    times = []
+    for _ in range(iters):
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+    return float(np.mean(times))
+
+
+def c1_leg(pkg, lib, torch, dev, sp, timer):
+    """Config 1: row-major 1024^3, A and B U[-1,1) from seed 0, C = 0, on the
+    oracle (the reference algorithm, all host cores, MaxIter 4 as
+    src/mtm.cpp:373) and on the GPU kernels (20 timed calls), with the
+    max- and Frobenius-relative errors against an FP64 product."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import helpers
+
+    n = 1024
+    log("config 1: mtm 1024^3 (seed 0), oracle and GPU")
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    a = helpers.rng_uniform(n * n, 0, 0).reshape(n, n)
+    b = helpers.rng_uniform(n * n, 0, 1).reshape(n, n)
+    truth = a.astype(np.float64) @ b.astype(np.float64)
+    blk = helpers.host_blocking()
+    c = np.zeros((n, n), np.float32)
+
+    def cpu_call():
+        c[:] = 0
+        helpers.oracle_mtm(c, a, b, blk, cores)
+
+    t_cpu = _mean_time(cpu_call, 4)
+    ref = helpers.rel_errors(c, truth)
+    flop = 2.0 * n ** 3
+    out = {"flop_2mnk": flop, "flop_reference_count": float(n * n * (2 * n - 1)),
+           "oracle": {"ms": t_cpu * 1e3, "tflops": flop / t_cpu / 1e12, "cores": cores,
+                      "max_rel": ref[0], "frob_rel": ref[1]}}
+    A = torch.from_numpy(a).to(dev)
+    B = torch.from_numpy(b).to(dev)
+    C = torch.zeros((n, n), device=dev)
+    torch.cuda.synchronize()
+    for mode, name in ((pkg.PLAIN_F32, "gpu_plain_f32"), (pkg.SPLIT_BF16, "gpu_split_bf16_p6")):
+        fn = lambda: lib.omb_sgemm(n, n, n, 1.0, A.data_ptr(), n, 1, B.data_ptr(), n, 1, 0.0,
+                                   C.data_ptr(), n, 1, mode, sp)
+        pkg.check(fn())
+        torch.cuda.synchronize()
+        err = helpers.rel_errors(C.cpu().numpy(), truth)
+        t = timer.run(fn, 20)
+        out[name] = {"us": t * 1e6, "tflops": flop / t / 1e12, "max_rel": err[0],
+                     "frob_rel": err[1], "ratio_vs_oracle": [err[0] / ref[0], err[1] / ref[1]]}
+    return out
+
+
+def cpu_reference_legs(pkg, torch, dev, stream):
+    """The oracle (the reference algorithm, OpenMP on all host cores) on the
+    bandwidth-bound configurations at full size: SGEMV 16384^2 in both
+    layouts (MaxIter 10, src/mtv.cpp:332) and SDOT 2^28 (MaxIter 100,
+    src/dot.cpp:198), inputs from seed 1 made by the device generator
+    (bit-identical to the host twin) and copied to the host."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import helpers
+
+    sp = stream.cuda_stream
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    blk = helpers.host_blocking()
+    out = {"cores": cores}
+    n = 16384
+    log("cpu reference path: oracle mtv 16384^2 (both layouts) and dot 2^28")
+    with torch.cuda.stream(stream):
+        Ad = torch.empty(n * n, device=dev)
+        xd = torch.empty(n, device=dev)
+    pkg.fill_uniform(Ad, 1, 0, stream=sp)
+    pkg.fill_uniform(xd, 1, 1, stream=sp)
+    torch.cuda.synchronize()
+    a = Ad.cpu().numpy().reshape(n, n)
+    x = xd.cpu().numpy()
+    del Ad
+    y = np.zeros(n, np.float32)
+    nbytes = 4.0 * (n * n + 2 * n)
+    t = _mean_time(lambda: helpers.oracle_mtv(y, a, x, blk, cores), 10)
+    out["mtv_row_major_16384"] = {"ms": t * 1e3, "gbps": nbytes / t / 1e9}
+    t = _mean_time(lambda: helpers.oracle_mtv(y, a.T, x, blk, cores), 10)   # column-major view
+    out["mtv_col_major_16384"] = {"ms": t * 1e3, "gbps": (nbytes + 4 * n) / t / 1e9}
+    del a
+    nd = 1 << 28
+    with torch.cuda.stream(stream):
+        xd = torch.empty(nd, device=dev)
+        yd = torch.empty(nd, device=dev)
+    pkg.fill_uniform(xd, 1, 0, stream=sp)
+    pkg.fill_uniform(yd, 1, 1, stream=sp)
+    torch.cuda.synchronize()
+    xh, yh = xd.cpu().numpy(), yd.cpu().numpy()
+    del xd, yd
+    t = _mean_time(lambda: helpers.oracle_dot(xh, yh, blk, cores), 100)
+    out["dot_2^28"] = {"ms": t * 1e3, "gbps": 8.0 * nd / t / 1e9}
     return out
 
 
