@@ -64,10 +64,21 @@ omb_status resolve_layout(const char* name, int64_t rows, int64_t cols, int64_t 
 // ------------------------------------------------------------ kernels -----
 // Plain FP32 SGEMM, row-major C (ldc), A K-contiguous iff a_kc, B N-contiguous
 // iff b_nc.  C = alpha*A*B + beta*C.
+// K splits for a GEMM whose `tiles` output tiles leave CUs idle (1: none)
+// and the k per split (a multiple of `align`): at least 256 k per split,
+// only for K >= 1024.  A launch with more than one split needs
+// splits * M * N floats of partial sums, from `part` when given, else from
+// the stream's workspace; launch_splitk_reduce adds them in split order.
+int64_t gemm_k_splits(int64_t tiles, int64_t K, int64_t align, int64_t* kchunk);
+int64_t sgemm_f32_splits(int64_t M, int64_t N, int64_t K, int64_t* kchunk);
+int64_t sgemm_split_splits(int64_t M, int64_t N, int64_t K, int64_t* kchunk);
+omb_status launch_splitk_reduce(int64_t M, int64_t N, int64_t splits, float alpha, float beta,
+                                const float* part, float* C, int64_t ldc, hipStream_t s);
 omb_status launch_sgemm_f32(int64_t M, int64_t N, int64_t K, float alpha,
                             const float* A, int64_t lda, bool a_kc,
                             const float* B, int64_t ldb, bool b_nc,
-                            float beta, float* C, int64_t ldc, hipStream_t s);
+                            float beta, float* C, int64_t ldc, hipStream_t s,
+                            float* part = nullptr);
 
 // Split-BF16 SGEMM (row-major C after layout folding); operands as strided
 // views: both are split into the packed form in the stream's workspace, then
@@ -83,7 +94,7 @@ omb_status launch_split_pack(int64_t R, int64_t K, const float* X, int64_t rs, i
                              __bf16* out, hipStream_t s);
 omb_status launch_sgemm_packed(int64_t M, int64_t N, int64_t K, float alpha, const __bf16* ap,
                                const __bf16* bp, float beta, float* C, int64_t ldc, int nprod,
-                               hipStream_t s);
+                               hipStream_t s, float* part = nullptr);
 
 // Scale C (row-major) by beta (K == 0 case).
 omb_status launch_scale(int64_t M, int64_t N, float beta, float* C, int64_t ldc, hipStream_t s);

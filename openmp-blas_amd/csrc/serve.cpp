@@ -92,11 +92,27 @@ class Arena {
     std::list<Block> free_;
 };
 
+// Packed split-BF16 operands of a GEMM request, rounded to 256 bytes (the
+// split partial sums, when K is split, follow them).
+size_t packed_bytes(const omb_request& r) {
+    return (size_t)round_up(
+        (split_packed_elems(r.m, r.k) + split_packed_elems(r.n, r.k)) * (int64_t)sizeof(__bf16), 256);
+}
+
 size_t need_bytes(const omb_request& r) {
     switch (r.kind) {
-        case OMB_REQ_GEMM:
-            if (r.mode == OMB_MODE_PLAIN_F32 || r.m == 0 || r.n == 0 || r.k == 0) return 0;
-            return (size_t)(split_packed_elems(r.m, r.k) + split_packed_elems(r.n, r.k)) * 2;
+        case OMB_REQ_GEMM: {
+            if (r.m == 0 || r.n == 0 || r.k == 0) return 0;
+            if (r.mode == OMB_MODE_PLAIN_F32) {  // partial sums when K is split
+                int64_t kc = 0;
+                const int64_t splits = sgemm_f32_splits(r.m, r.n, r.k, &kc);
+                return splits > 1 ? (size_t)(splits * r.m * r.n) * sizeof(float) : 0;
+            }
+            int64_t kc = 0;
+            const int64_t splits = sgemm_split_splits(r.m, r.n, r.k, &kc);
+            return packed_bytes(r) +
+                   (splits > 1 ? (size_t)(splits * r.m * r.n) * sizeof(float) : 0);
+        }
         case OMB_REQ_DOT: return r.n > 0 ? (size_t)sdot_blocks(r.n) * sizeof(double) : 0;
         default: return 0;
     }
@@ -140,13 +156,14 @@ omb_status launch_request(const omb_request& r, char* ws, hipStream_t s) {
             if (r.k == 0) return launch_scale(r.m, r.n, 0.f, r.c, r.n, s);
             if (r.mode == OMB_MODE_PLAIN_F32)
                 return launch_sgemm_f32(r.m, r.n, r.k, 1.f, r.a, r.k, true, r.b, r.n, true, 0.f, r.c,
-                                        r.n, s);
+                                        r.n, s, reinterpret_cast<float*>(ws));
             __bf16* ap = reinterpret_cast<__bf16*>(ws);
             __bf16* bp = ap + split_packed_elems(r.m, r.k);
             OMB_TRY(launch_split_pack(r.m, r.k, r.a, r.k, 1, ap, s));
             OMB_TRY(launch_split_pack(r.n, r.k, r.b, 1, r.n, bp, s));  // B^T of row-major B
             return launch_sgemm_packed(r.m, r.n, r.k, 1.f, ap, bp, 0.f, r.c, r.n,
-                                       r.mode == OMB_MODE_SPLIT_BF16_P3 ? 3 : 6, s);
+                                       r.mode == OMB_MODE_SPLIT_BF16_P3 ? 3 : 6, s,
+                                       reinterpret_cast<float*>(ws + packed_bytes(r)));
         }
         case OMB_REQ_GEMV:
             if (r.m == 0) return OMB_OK;

@@ -26,6 +26,8 @@ constexpr int NT = 256;         // threads per workgroup
 constexpr int INNER_K = 128;    // k steps per inner accumulation block
 constexpr int PAD = 4;          // LDS row padding (floats), keeps float4 alignment
 constexpr int kGroupM = 8;      // tile rows per L2 group
+constexpr int64_t kSplitMinK = 256;   // least k per split when K is split
+constexpr int64_t kSplitFromK = 1024;  // K is split only from this length on
 
 struct Args {
     int64_t M, N, K;
@@ -37,6 +39,8 @@ struct Args {
     float* C;
     int64_t ldc;
     int64_t tiles_m, tiles_n;
+    int64_t kchunk;  // k per split (blockIdx.y), a multiple of INNER_K
+    float* part;     // split partial sums [split][M][N], or null (one split)
 };
 
 // Load up to 4 consecutive floats starting at p (n valid, 0..4); zero-fill.
@@ -117,9 +121,21 @@ __device__ inline void store_slice(float (*As)[BM + PAD], float (*Bs)[BN + PAD],
 }
 
 template <bool A_KC, bool B_NC, bool VEC>
-__global__ __launch_bounds__(NT, 2) void sgemm_f32_kernel(Args p) {
+__global__ __launch_bounds__(NT, 2) void sgemm_f32_kernel(Args q) {
     __shared__ float As[2][BK][BM + PAD];
     __shared__ float Bs[2][BK][BN + PAD];
+
+    // Split blockIdx.y takes k in [y * kchunk, min(K, (y + 1) * kchunk)): the
+    // operands are offset to that range and K becomes its length.  kchunk is
+    // a multiple of INNER_K, so the 128-k accumulation blocks are the same
+    // as without the split.
+    Args p = q;
+    const int64_t kbeg = (int64_t)blockIdx.y * q.kchunk;
+    if (q.part) {
+        p.K = q.K - kbeg < q.kchunk ? q.K - kbeg : q.kchunk;
+        p.A = q.A + (A_KC ? kbeg : kbeg * q.lda);
+        p.B = q.B + (B_NC ? kbeg * q.ldb : kbeg);
+    }
 
     const int tid = threadIdx.x;
     const int64_t ntiles = p.tiles_m * p.tiles_n;
@@ -188,6 +204,29 @@ __global__ __launch_bounds__(NT, 2) void sgemm_f32_kernel(Args p) {
     }
 
     // Epilogue: rows {ty*4+i, 64+ty*4+i}, columns {tx*4+j, 64+tx*4+j}.
+    if (p.part) {  // raw partial sums of this split, row-major M x N
+        float* part = p.part + (int64_t)blockIdx.y * p.M * p.N;
+        const bool pvec = p.N % 4 == 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int64_t m = m0 + (i < 4 ? ty * 4 + i : 64 + ty * 4 + (i - 4));
+            if (m >= p.M) continue;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int64_t n = n0 + (h == 0 ? tx * 4 : 64 + tx * 4);
+                float* c = part + m * p.N + n;
+                if (pvec && n + 3 < p.N) {
+                    *reinterpret_cast<float4*>(c) = make_float4(outer[i][h * 4], outer[i][h * 4 + 1],
+                                                                outer[i][h * 4 + 2], outer[i][h * 4 + 3]);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        if (n + j < p.N) c[j] = outer[i][h * 4 + j];
+                }
+            }
+        }
+        return;
+    }
     const bool cvec = ((reinterpret_cast<uintptr_t>(p.C) & 15) == 0) && (p.ldc % 4 == 0);
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
@@ -220,9 +259,26 @@ __global__ __launch_bounds__(NT, 2) void sgemm_f32_kernel(Args p) {
     }
 }
 
+// C = alpha * (sum of the splits' partials, in split order) + beta * C.
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(int64_t M, int64_t N, int64_t splits,
+                                                            float alpha, float beta,
+                                                            const float* __restrict__ part,
+                                                            float* C, int64_t ldc) {
+    const int64_t total = M * N;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        float s = part[i];
+        for (int64_t t = 1; t < splits; ++t) s += part[t * total + i];
+        const int64_t m = i / N, n = i - m * N;
+        float* c = C + m * ldc + n;
+        const float v = alpha * s;
+        *c = beta != 0.f ? v + beta * *c : v;
+    }
+}
+
 template <bool A_KC, bool B_NC>
-void launch_variant(const Args& a, bool vec, hipStream_t s) {
-    dim3 grid((unsigned)(a.tiles_m * a.tiles_n)), block(NT);
+void launch_variant(const Args& a, int64_t splits, bool vec, hipStream_t s) {
+    dim3 grid((unsigned)(a.tiles_m * a.tiles_n), (unsigned)splits), block(NT);
     if (vec)
         hipLaunchKernelGGL((sgemm_f32_kernel<A_KC, B_NC, true>), grid, block, 0, s, a);
     else
@@ -241,23 +297,64 @@ __global__ void scale_kernel(int64_t M, int64_t N, float beta, float* C, int64_t
 
 }  // namespace
 
+int64_t gemm_k_splits(int64_t tiles, int64_t K, int64_t align, int64_t* kchunk) {
+    *kchunk = K;
+    const int64_t cus = device_cu_count();
+    if (tiles <= 0 || tiles >= cus || K < kSplitFromK) return 1;
+    int64_t splits = std::min(cus / tiles, K / kSplitMinK);
+    if (splits < 2) return 1;
+    const int64_t kc = round_up(ceil_div(K, splits), align);
+    splits = ceil_div(K, kc);
+    if (splits < 2) return 1;
+    *kchunk = kc;
+    return splits;
+}
+
+int64_t sgemm_f32_splits(int64_t M, int64_t N, int64_t K, int64_t* kchunk) {
+    return gemm_k_splits(ceil_div(M, BM) * ceil_div(N, BN), K, INNER_K, kchunk);
+}
+
+omb_status launch_splitk_reduce(int64_t M, int64_t N, int64_t splits, float alpha, float beta,
+                                const float* part, float* C, int64_t ldc, hipStream_t s) {
+    const unsigned blocks =
+        (unsigned)std::min<int64_t>(ceil_div(M * N, 256), 4 * device_cu_count());
+    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(blocks), dim3(256), 0, s, M, N, splits, alpha,
+                       beta, part, C, ldc);
+    OMB_HIP_TRY(hipGetLastError());
+    return OMB_OK;
+}
+
 omb_status launch_sgemm_f32(int64_t M, int64_t N, int64_t K, float alpha,
                             const float* A, int64_t lda, bool a_kc,
                             const float* B, int64_t ldb, bool b_nc,
-                            float beta, float* C, int64_t ldc, hipStream_t s) {
-    Args a{M, N, K, alpha, beta, A, lda, B, ldb, C, ldc, ceil_div(M, BM), ceil_div(N, BN)};
-    if (a.tiles_m * a.tiles_n > 0x7fffffffLL) {
+                            float beta, float* C, int64_t ldc, hipStream_t s, float* part) {
+    Args a{M, N, K, alpha, beta, A, lda, B, ldb, C, ldc, ceil_div(M, BM), ceil_div(N, BN), K,
+           nullptr};
+    const int64_t tiles = a.tiles_m * a.tiles_n;
+    if (tiles > 0x7fffffffLL) {
         set_error("sgemm: too many tiles for one launch");
         return OMB_EUNSUPPORTED;
+    }
+    // Split K over blockIdx.y when the tiles alone leave CUs idle; the
+    // splits' partials are then added in split order.
+    const int64_t splits = sgemm_f32_splits(M, N, K, &a.kchunk);
+    if (splits > 1) {
+        if (!part) {
+            Workspace* ws = nullptr;
+            OMB_TRY(get_workspace(s, (size_t)(splits * M * N) * sizeof(float), &ws));
+            part = reinterpret_cast<float*>(ws->buf);
+        }
+        a.part = part;
     }
     const bool vec = ((reinterpret_cast<uintptr_t>(A) & 15) == 0) &&
                      ((reinterpret_cast<uintptr_t>(B) & 15) == 0) && (lda % 4 == 0) &&
                      (ldb % 4 == 0);
-    if (a_kc && b_nc) launch_variant<true, true>(a, vec, s);
-    else if (a_kc && !b_nc) launch_variant<true, false>(a, vec, s);
-    else if (!a_kc && b_nc) launch_variant<false, true>(a, vec, s);
-    else launch_variant<false, false>(a, vec, s);
+    if (a_kc && b_nc) launch_variant<true, true>(a, splits, vec, s);
+    else if (a_kc && !b_nc) launch_variant<true, false>(a, splits, vec, s);
+    else if (!a_kc && b_nc) launch_variant<false, true>(a, splits, vec, s);
+    else launch_variant<false, false>(a, splits, vec, s);
     OMB_HIP_TRY(hipGetLastError());
+    if (splits > 1) return launch_splitk_reduce(M, N, splits, alpha, beta, a.part, C, ldc, s);
     return OMB_OK;
 }
 

@@ -134,6 +134,8 @@ struct MArgs {
     int64_t M, N;
     float alpha, beta;
     int64_t tiles_m, tiles_n;
+    int64_t ktc;  // slices per split (blockIdx.y), a multiple of kFoldSlices
+    float* part;  // split partial sums [split][M][N], or null (one split)
 };
 
 constexpr int VA = SLICE_A / 8 / NT;  // 16-byte loads per thread per slice (3)
@@ -181,8 +183,12 @@ __global__ __launch_bounds__(NT, 2) void sgemm_bf16x3_kernel(MArgs p) {
     int64_t tm, tn;
     group_tile(t, p.tiles_m, p.tiles_n, kGroupM, &tm, &tn);
 
-    const uint4* ga = reinterpret_cast<const uint4*>(p.Ap + tm * p.nkt * SLICE_A);
-    const uint4* gb = reinterpret_cast<const uint4*>(p.Bp + tn * p.nkt * SLICE_B);
+    // Split blockIdx.y takes slices [y * ktc, min(nkt, (y + 1) * ktc)) of the
+    // tile's row and column panels; ktc is a multiple of kFoldSlices, so the
+    // accumulation blocks are the same as without the split.
+    const int64_t kt0 = (int64_t)blockIdx.y * p.ktc;
+    const uint4* ga = reinterpret_cast<const uint4*>(p.Ap + (tm * p.nkt + kt0) * SLICE_A);
+    const uint4* gb = reinterpret_cast<const uint4*>(p.Bp + (tn * p.nkt + kt0) * SLICE_B);
     static_assert(VA == 3 && VB == 3, "staging macros assume 3 + 3 loads per slice");
     // LDS slots of this thread's chunks (loop invariant).
     const int q0 = lds_index(tid), q1 = lds_index(tid + NT), q2 = lds_index(tid + 2 * NT);
@@ -197,10 +203,10 @@ __global__ __launch_bounds__(NT, 2) void sgemm_bf16x3_kernel(MArgs p) {
     auto compute = [&](int buf, int64_t kt) {
         mfma_slice<NPROD>(lds[buf], wm * 64, wn * 64, frow, fh, inner);
     };
+    const int64_t nkt = p.nkt - kt0 < p.ktc ? p.nkt - kt0 : p.ktc;
     auto fold_at = [&](int64_t kt) {
-        if ((kt + 1) % kFoldSlices == 0 || kt + 1 == p.nkt) fold(inner, outer);
+        if ((kt + 1) % kFoldSlices == 0 || kt + 1 == nkt) fold(inner, outer);
     };
-    const int64_t nkt = p.nkt;
     uint4 x0, x1, x2, x3, x4, x5;  // even slices in flight
     uint4 y0, y1, y2, y3, y4, y5;  // odd slices in flight
     OMB_GLOAD(0, x0, x1, x2, x3, x4, x5);
@@ -223,8 +229,12 @@ __global__ __launch_bounds__(NT, 2) void sgemm_bf16x3_kernel(MArgs p) {
         __syncthreads();
     }
 
-    store_tile(outer, tm * TM + wm * 64, tn * TN + wn * 64, lane, p.C, p.ldc, p.M, p.N,
-               p.alpha, p.beta);
+    if (p.part)  // raw partial sums of this split (alpha 1, beta 0 store them unchanged)
+        store_tile(outer, tm * TM + wm * 64, tn * TN + wn * 64, lane,
+                   p.part + (int64_t)blockIdx.y * p.M * p.N, p.N, p.M, p.N, 1.f, 0.f);
+    else
+        store_tile(outer, tm * TM + wm * 64, tn * TN + wn * 64, lane, p.C, p.ldc, p.M, p.N,
+                   p.alpha, p.beta);
 }
 
 }  // namespace
@@ -250,21 +260,40 @@ omb_status launch_split_pack(int64_t R, int64_t K, const float* X, int64_t rs, i
     return OMB_OK;
 }
 
+int64_t sgemm_split_splits(int64_t M, int64_t N, int64_t K, int64_t* kchunk) {
+    return gemm_k_splits(ceil_div(M, TM) * ceil_div(N, TN), K, TK * kFoldSlices, kchunk);
+}
+
 omb_status launch_sgemm_packed(int64_t M, int64_t N, int64_t K, float alpha, const __bf16* ap,
                                const __bf16* bp, float beta, float* C, int64_t ldc, int nprod,
-                               hipStream_t s) {
+                               hipStream_t s, float* part) {
     const int64_t tiles_m = ceil_div(M, TM), tiles_n = ceil_div(N, TN);
     if (tiles_m * tiles_n > 0x7fffffffLL) {
         set_error("sgemm split: too many tiles for one launch");
         return OMB_EUNSUPPORTED;
     }
-    MArgs a{ap, bp, ceil_div(K, TK), C, ldc, M, N, alpha, beta, tiles_m, tiles_n};
-    dim3 grid((unsigned)(tiles_m * tiles_n)), block(NT);
+    const int64_t nkt = ceil_div(K, TK);
+    MArgs a{ap, bp, nkt, C, ldc, M, N, alpha, beta, tiles_m, tiles_n, nkt, nullptr};
+    // Split K over blockIdx.y when the tiles alone leave CUs idle; the
+    // splits' partials are then added in split order.
+    int64_t kchunk = K;
+    const int64_t splits = sgemm_split_splits(M, N, K, &kchunk);
+    if (splits > 1) {
+        if (!part) {
+            Workspace* ws = nullptr;
+            OMB_TRY(get_workspace(s, (size_t)(splits * M * N) * sizeof(float), &ws));
+            part = reinterpret_cast<float*>(ws->buf);
+        }
+        a.ktc = kchunk / TK;
+        a.part = part;
+    }
+    dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)splits), block(NT);
     if (nprod == 3)
         hipLaunchKernelGGL(sgemm_bf16x3_kernel<3>, grid, block, 0, s, a);
     else
         hipLaunchKernelGGL(sgemm_bf16x3_kernel<6>, grid, block, 0, s, a);
     OMB_HIP_TRY(hipGetLastError());
+    if (splits > 1) return launch_splitk_reduce(M, N, splits, alpha, beta, part, C, ldc, s);
     return OMB_OK;
 }
 
@@ -274,14 +303,20 @@ omb_status launch_sgemm_split(int64_t M, int64_t N, int64_t K, float alpha,
                               float beta, float* C, int64_t ldc, int nprod, hipStream_t s) {
     const size_t a_elems = (size_t)split_packed_elems(M, K);
     const size_t b_elems = (size_t)split_packed_elems(N, K);
+    // one workspace request for the packed operands and any split partials
+    const size_t packed = round_up((int64_t)((a_elems + b_elems) * sizeof(__bf16)), 256);
+    int64_t kchunk = K;
+    const int64_t splits = sgemm_split_splits(M, N, K, &kchunk);
+    const size_t partial = splits > 1 ? (size_t)(splits * M * N) * sizeof(float) : 0;
     Workspace* ws = nullptr;
-    OMB_TRY(get_workspace(s, (a_elems + b_elems) * sizeof(__bf16), &ws));
+    OMB_TRY(get_workspace(s, packed + partial, &ws));
     __bf16* ap = reinterpret_cast<__bf16*>(ws->buf);
     __bf16* bp = ap + a_elems;
+    float* part = partial ? reinterpret_cast<float*>(static_cast<char*>(ws->buf) + packed) : nullptr;
     // A as rows m, columns k; B^T as rows n, columns k (element (n,k) = B(k,n)).
     OMB_TRY(launch_split_pack(M, K, A, rsA, csA, ap, s));
     OMB_TRY(launch_split_pack(N, K, B, csB, rsB, bp, s));
-    return launch_sgemm_packed(M, N, K, alpha, ap, bp, beta, C, ldc, nprod, s);
+    return launch_sgemm_packed(M, N, K, alpha, ap, bp, beta, C, ldc, nprod, s, part);
 }
 
 }  // namespace omb

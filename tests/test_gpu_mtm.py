@@ -104,6 +104,47 @@ def test_mtm_alpha_beta_and_k_zero(dev, ompblas):
     assert torch.equal(c, 3.0 * c0)
 
 
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("combo", ["LLL", "FFF", "LFL", "FLF"])
+def test_split_k_integer_exact(dev, ompblas, combo, mode):
+    """Few output tiles and a long K: K is split over workgroups and the
+    partial sums added in order.  Values 0..9 keep every sum exact (K * 81 <
+    2^24), so each split count, layout and alpha/beta gives the exact product."""
+    order = {"F": "F", "L": "C"}
+    oc, oa, ob = (order[ch] for ch in combo)
+    for m, n, k in [(200, 300, 3000), (1, 129, 4100), (257, 5, 1024), (128, 128, 20000)]:
+        a = (rand100((m, k), oa) % 10).astype(np.float32)
+        b = (rand100((k, n), ob) % 10).astype(np.float32)
+        c0 = rand100((m, n), oc)
+        want = 2.0 * (a.astype(np.float64) @ b.astype(np.float64)) + 0.5 * c0
+        tc = to_dev(c0, dev)
+        run_mtm(ompblas, tc, to_dev(a, dev), to_dev(b, dev), mode, alpha=2.0, beta=0.5)
+        assert np.array_equal(tc.cpu().numpy().astype(np.float64), want), (combo, mode, m, n, k)
+
+
+def test_split_k_deterministic_and_within_error_bound(dev, ompblas):
+    """A split-K product (256 x 192 x 8192, U[-1,1)) repeats bit for bit and
+    keeps the <= 2x error bound against the oracle, in every mode that
+    claims it."""
+    M, N, K = 256, 192, 8192
+    a = helpers.rng_uniform(M * K, 7, 0).reshape(M, K)
+    b = helpers.rng_uniform(K * N, 7, 1).reshape(K, N)
+    truth = a.astype(np.float64) @ b.astype(np.float64)
+    c_ref = np.zeros((M, N), np.float32)
+    oracle_mtm(c_ref, a, b)
+    ref = rel_errors(c_ref, truth)
+    ta, tb = to_dev(a, dev), to_dev(b, dev)
+    for mode in (0, 1):
+        outs = []
+        for _ in range(2):
+            tc = torch.zeros(M, N, device=dev)
+            run_mtm(ompblas, tc, ta, tb, mode)
+            outs.append(tc)
+        assert torch.equal(outs[0], outs[1]), mode
+        got = rel_errors(outs[0].cpu().numpy(), truth)
+        assert got[0] <= 2 * ref[0] and got[1] <= 2 * ref[1], (mode, got, ref)
+
+
 @pytest.mark.parametrize("orders", ["CC", "FF", "CF"])
 def test_nonfinite_inputs(dev, ompblas, orders):
     """Plain FP32 follows IEEE exactly as the reference's FP32 loops do: an Inf

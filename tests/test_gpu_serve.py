@@ -128,6 +128,23 @@ def test_small_budget_makes_requests_wait(dev, ompblas):
         assert torch.equal(mixed[i], tr.solo(i)), i
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_split_k_requests_equal_solo(dev, ompblas, mode):
+    """GEMMs with few output tiles and long K take the split-K path, whose
+    partial sums live in the request's workspace from the arena; overlapped
+    in the loop they give the same bits as the direct entry point."""
+    trace = [(ompblas.REQ_GEMM, 64 + 40 * i, 96 + 32 * i, 1024 + 700 * i, 0.0) for i in range(8)]
+    trace += [(ompblas.REQ_DOT, 0, 70000 + i, 0, 0.0) for i in range(4)]
+    tr = Trace(dev, ompblas, trace, seed=64, mode=mode)
+    res, st = ompblas.serve_run(tr.requests(), slots=6)
+    torch.cuda.synchronize()
+    assert all(r["status"] == 0 for r in res)
+    assert st["max_in_flight"] > 1, st
+    mixed = tr.outputs()
+    for i in range(len(trace)):
+        assert torch.equal(mixed[i], tr.solo(i)), (i, trace[i])
+
+
 def test_oversize_request_is_refused(dev, ompblas):
     """A request whose workspace exceeds the budget gets EBUSY; the rest run."""
     trace = [(ompblas.REQ_GEMM, 512, 512, 512, 0.0), (ompblas.REQ_GEMV, 300, 400, 0, 0.0),
