@@ -240,6 +240,11 @@ typedef struct {
 
 omb_status omb_serve_run(omb_request* reqs, int32_t n, const omb_serve_config* cfg,
                          omb_request_result* results, omb_serve_stats* stats);
+/* The workspace bytes omb_serve_run will hold for request r while it runs
+ * (the value it fills into ws_bytes), for sizing hbm_budget; -1 for an
+ * invalid request.  Split-BF16 GEMMs hold their packed operands, GEMMs
+ * whose K is split hold the partial sums, dots their partials. */
+int64_t omb_serve_request_bytes(const omb_request* r);
 
 /* ----------------------------------------------- host-pointer helpers ----
  * Used by the C++ drop-in layer to accept the reference's host buffers. */

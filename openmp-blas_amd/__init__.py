@@ -130,6 +130,7 @@ _SIGS = {
     "omb_group_plan_destroy": (_I32, [_VP]),
     "omb_autotune_group": (_I32, [ctypes.POINTER(GemmDesc), _I32, _I32, ctypes.c_char_p, _VP]),
     "omb_autotune_load": (_I32, [ctypes.c_char_p]),
+    "omb_serve_request_bytes": (_I64, [ctypes.POINTER(Request)]),
     "omb_serve_run": (_I32, [ctypes.POINTER(Request), _I32, ctypes.POINTER(ServeConfig),
                              ctypes.POINTER(RequestResult), ctypes.POINTER(ServeStats)]),
     "omb_is_device_pointer": (_I32, [_VP]),
@@ -466,6 +467,14 @@ def serve_run(requests, slots: int = 8, hbm_budget: int = 0, hbm_reserve: int = 
                         status=r.status, slot=r.slot, ws_bytes=reqs[i].ws_bytes))
     stats = {name: getattr(st, name) for name, _ in ServeStats._fields_}
     return out, stats
+
+
+def serve_request_bytes(request) -> int:
+    """Workspace bytes omb_serve_run holds for `request` while it runs."""
+    v = lib().omb_serve_request_bytes(ctypes.byref(request))
+    if v < 0:
+        raise OmbError(EINVAL, last_error())
+    return int(v)
 
 
 def log_uniform_ints(rng, lo: int, hi: int, size):

@@ -64,12 +64,18 @@ omb_status resolve_layout(const char* name, int64_t rows, int64_t cols, int64_t 
 // ------------------------------------------------------------ kernels -----
 // Plain FP32 SGEMM, row-major C (ldc), A K-contiguous iff a_kc, B N-contiguous
 // iff b_nc.  C = alpha*A*B + beta*C.
-// K splits for a GEMM whose `tiles` output tiles leave CUs idle (1: none)
-// and the k per split (a multiple of `align`): at least 256 k per split,
-// only for K >= 1024.  A launch with more than one split needs
-// splits * M * N floats of partial sums, from `part` when given, else from
-// the stream's workspace; launch_splitk_reduce adds them in split order.
-int64_t gemm_k_splits(int64_t tiles, int64_t K, int64_t align, int64_t* kchunk);
+// K splits for an M x N x K GEMM of 128 x 128 tiles (1: none) and the k per
+// split, a multiple of `align` (the kernel's accumulation block).  Up to
+// K = 512, and only when the tiles leave CUs idle, each split is one block,
+// which gives the unsplit kernel's bits.  Beyond, at least 256 k per split,
+// and the count that minimises a model of waves x k per workgroup (at
+// `tile_rate` flop/s per workgroup) plus the partials' traffic:
+// deterministic, with a different rounding.  A launch with more than one
+// split needs splits * M * N floats of partial sums, from `part` when given,
+// else from the stream's workspace; launch_splitk_reduce adds them in split
+// order.
+int64_t gemm_k_splits(int64_t M, int64_t N, int64_t K, int64_t align, double tile_rate,
+                      int64_t* kchunk);
 int64_t sgemm_f32_splits(int64_t M, int64_t N, int64_t K, int64_t* kchunk);
 int64_t sgemm_split_splits(int64_t M, int64_t N, int64_t K, int64_t* kchunk);
 omb_status launch_splitk_reduce(int64_t M, int64_t N, int64_t splits, float alpha, float beta,

@@ -363,3 +363,12 @@ extern "C" omb_status omb_serve_run(omb_request* reqs, int32_t n, const omb_serv
     }
     return OMB_OK;
 }
+
+extern "C" int64_t omb_serve_request_bytes(const omb_request* r) {
+    clear_error();
+    if (!r || check_request(*r, 0) != OMB_OK) {
+        set_error("omb_serve_request_bytes: invalid request");
+        return -1;
+    }
+    return (int64_t)need_bytes(*r);
+}

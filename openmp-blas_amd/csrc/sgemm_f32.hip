@@ -26,8 +26,12 @@ constexpr int NT = 256;         // threads per workgroup
 constexpr int INNER_K = 128;    // k steps per inner accumulation block
 constexpr int PAD = 4;          // LDS row padding (floats), keeps float4 alignment
 constexpr int kGroupM = 8;      // tile rows per L2 group
-constexpr int64_t kSplitMinK = 256;   // least k per split when K is split
-constexpr int64_t kSplitFromK = 1024;  // K is split only from this length on
+constexpr int64_t kSplitMinK = 256;    // least k per split for K > kSplitExactK
+constexpr int64_t kSplitExactK = 512;  // up to this K, splits are single accumulation blocks
+constexpr int64_t kSplitMost = 16;     // most splits considered
+constexpr double kTileRateF32 = 0.33e12;  // FP32 flop/s of one 128x128 workgroup (measured)
+constexpr double kSplitHbm = 4.0e12;      // bytes/s for the partials' traffic
+constexpr double kSplitLaunch = 4.0e-6;   // the reduction kernel's launch and tail
 
 struct Args {
     int64_t M, N, K;
@@ -297,21 +301,47 @@ __global__ void scale_kernel(int64_t M, int64_t N, float beta, float* C, int64_t
 
 }  // namespace
 
-int64_t gemm_k_splits(int64_t tiles, int64_t K, int64_t align, int64_t* kchunk) {
+int64_t gemm_k_splits(int64_t M, int64_t N, int64_t K, int64_t align, double tile_rate,
+                      int64_t* kchunk) {
     *kchunk = K;
+    const int64_t tiles = ceil_div(M, 128) * ceil_div(N, 128);
     const int64_t cus = device_cu_count();
-    if (tiles <= 0 || tiles >= cus || K < kSplitFromK) return 1;
-    int64_t splits = std::min(cus / tiles, K / kSplitMinK);
-    if (splits < 2) return 1;
-    const int64_t kc = round_up(ceil_div(K, splits), align);
-    splits = ceil_div(K, kc);
-    if (splits < 2) return 1;
-    *kchunk = kc;
-    return splits;
+    if (tiles <= 0 || K <= align) return 1;
+    if (K <= kSplitExactK) {
+        if (tiles >= cus) return 1;
+        // One accumulation block per split: the in-order sum of the
+        // partials is the unsplit kernel's own fold, so the bits are equal.
+        *kchunk = align;
+        return ceil_div(K, align);
+    }
+    // Waves of workgroups times the k each one covers, plus the partials'
+    // write and read and the reduction launch: the split count with the
+    // least modelled time (1 when splitting does not pay).
+    auto cost = [&](int64_t splits, int64_t kc) {
+        const double waves = (double)ceil_div(tiles * splits, cus);
+        double t = waves * 2.0 * 128.0 * 128.0 * (double)kc / tile_rate;
+        if (splits > 1) t += (double)(splits + 1) * (double)(M * N) * 4.0 / kSplitHbm + kSplitLaunch;
+        return t;
+    };
+    int64_t best = 1;
+    double best_t = cost(1, K);
+    const int64_t most = std::min<int64_t>(kSplitMost, K / kSplitMinK);
+    for (int64_t sp = 2; sp <= most; ++sp) {
+        const int64_t kc = round_up(ceil_div(K, sp), align);
+        const int64_t n = ceil_div(K, kc);
+        const double t = cost(n, kc);
+        if (n > 1 && t < 0.95 * best_t) {
+            best = n;
+            best_t = t;
+            *kchunk = kc;
+        }
+    }
+    if (best == 1) *kchunk = K;
+    return best;
 }
 
 int64_t sgemm_f32_splits(int64_t M, int64_t N, int64_t K, int64_t* kchunk) {
-    return gemm_k_splits(ceil_div(M, BM) * ceil_div(N, BN), K, INNER_K, kchunk);
+    return gemm_k_splits(M, N, K, INNER_K, kTileRateF32, kchunk);
 }
 
 omb_status launch_splitk_reduce(int64_t M, int64_t N, int64_t splits, float alpha, float beta,

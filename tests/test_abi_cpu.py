@@ -147,6 +147,25 @@ def test_serve_run_rejects_bad_arguments(ompblas):
     assert "omb_serve_run" in ompblas.last_error()
 
 
+def test_serve_request_bytes(ompblas):
+    """Workspace a request holds in the loop: split GEMMs their packed
+    operands (rounded to 256 B), plain GEMMs with few tiles and K up to 512
+    one partial plane per 128-k block, dots their partials; bad requests
+    are refused."""
+    R, lib = ompblas.Request, ompblas.lib()
+    pack = lib.omb_split_pack_bytes(300, 260) + lib.omb_split_pack_bytes(200, 260)
+    split = ompblas.serve_request_bytes(R(ompblas.REQ_GEMM, 1, 300, 200, 260, 0.0, 16, 16, 16, 0))
+    assert split >= pack and split % 256 == 0
+    # 3 x 2 tiles, K = 500: four single-block splits of 300 x 200 floats
+    plain = ompblas.serve_request_bytes(R(ompblas.REQ_GEMM, 0, 300, 200, 500, 0.0, 16, 16, 16, 0))
+    assert plain == 4 * 300 * 200 * 4
+    assert ompblas.serve_request_bytes(R(ompblas.REQ_GEMM, 0, 300, 200, 100, 0.0, 16, 16, 16, 0)) == 0
+    assert ompblas.serve_request_bytes(R(ompblas.REQ_GEMV, 0, 30, 20, 0, 0.0, 16, 16, 16, 0)) == 0
+    assert ompblas.serve_request_bytes(R(ompblas.REQ_DOT, 0, 0, 5000, 0, 0.0, 16, 16, 16, 0)) > 0
+    with pytest.raises(ompblas.OmbError):
+        ompblas.serve_request_bytes(R(ompblas.REQ_GEMM, 9, 4, 4, 4, 0.0, 16, 16, 16, 0))
+
+
 def test_serving_trace_mix_and_ranges(ompblas):
     """The Poisson trace: seeded, sorted arrivals, the 50/30/20 mix and the
     log-uniform size ranges."""

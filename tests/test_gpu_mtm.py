@@ -122,6 +122,26 @@ def test_split_k_integer_exact(dev, ompblas, combo, mode):
         assert np.array_equal(tc.cpu().numpy().astype(np.float64), want), (combo, mode, m, n, k)
 
 
+@pytest.mark.parametrize("mode", MODES)
+def test_split_k_up_to_512_gives_the_unsplit_bits(dev, ompblas, mode):
+    """Up to K = 512 a split product splits K into single accumulation
+    blocks, whose in-order sum is the unsplit kernel's own fold: its rows
+    equal, bit for bit, the same rows of a taller product (enough tiles that
+    it does not split)."""
+    N = 300
+    big_m = 128 * 100                      # 100 x 3 tiles: more than the CUs, no split
+    for m, k in [(200, 500), (1, 300), (129, 512), (77, 129)]:
+        a_big = torch.empty(big_m, k, device=dev)
+        ompblas.fill_uniform(a_big, 11, 0)
+        b = torch.empty(k, N, device=dev)
+        ompblas.fill_uniform(b, 11, 1)
+        c_big = torch.zeros(big_m, N, device=dev)
+        run_mtm(ompblas, c_big, a_big, b, mode)
+        c = torch.zeros(m, N, device=dev)
+        run_mtm(ompblas, c, a_big[:m], b, mode)
+        assert torch.equal(c, c_big[:m]), (mode, m, k)
+
+
 def test_split_k_deterministic_and_within_error_bound(dev, ompblas):
     """A split-K product (256 x 192 x 8192, U[-1,1)) repeats bit for bit and
     keeps the <= 2x error bound against the oracle, in every mode that

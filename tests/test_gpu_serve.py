@@ -117,8 +117,9 @@ def test_small_budget_makes_requests_wait(dev, ompblas):
     trace = [(ompblas.REQ_GEMM, 300 + 7 * i, 200 + 5 * i, 260 + 3 * i, 0.0) for i in range(12)]
     trace += [(ompblas.REQ_DOT, 0, 5000 + i, 0, 0.0) for i in range(4)]
     tr = Trace(dev, ompblas, trace, seed=24, mode=1)
-    need = max(ompblas.lib().omb_split_pack_bytes(m, k) + ompblas.lib().omb_split_pack_bytes(n, k)
-               for _, m, n, k, _ in trace[:12])
+    need = max(ompblas.serve_request_bytes(r) for r in tr.requests()[:12])
+    assert need >= max(ompblas.lib().omb_split_pack_bytes(m, k) +
+                       ompblas.lib().omb_split_pack_bytes(n, k) for _, m, n, k, _ in trace[:12])
     res, st = ompblas.serve_run(tr.requests(), slots=8, hbm_budget=need + 4096)
     torch.cuda.synchronize()
     assert all(r["status"] == 0 for r in res)
