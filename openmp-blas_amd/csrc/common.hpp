@@ -68,14 +68,14 @@ omb_status resolve_layout(const char* name, int64_t rows, int64_t cols, int64_t 
 // split, a multiple of `align` (the kernel's accumulation block).  Up to
 // K = 512, and only when the tiles leave CUs idle, each split is one block,
 // which gives the unsplit kernel's bits.  Beyond, at least 256 k per split,
-// and the count that minimises a model of waves x k per workgroup (at
-// `tile_rate` flop/s per workgroup) plus the partials' traffic:
-// deterministic, with a different rounding.  A launch with more than one
-// split needs splits * M * N floats of partial sums, from `part` when given,
-// else from the stream's workspace; launch_splitk_reduce adds them in split
-// order.
-int64_t gemm_k_splits(int64_t M, int64_t N, int64_t K, int64_t align, double tile_rate,
-                      int64_t* kchunk);
+// and the count that minimises a model of rounds x k per workgroup (at
+// `rate_full` flop/s per workgroup with two per CU, `rate_alone` with one)
+// plus the partials' traffic: deterministic, with a different rounding.  A
+// launch with more than one split needs splits * M * N floats of partial
+// sums, from `part` when given, else from the stream's workspace;
+// launch_splitk_reduce adds them in split order.
+int64_t gemm_k_splits(int64_t M, int64_t N, int64_t K, int64_t align, double rate_full,
+                      double rate_alone, int64_t* kchunk);
 int64_t sgemm_f32_splits(int64_t M, int64_t N, int64_t K, int64_t* kchunk);
 int64_t sgemm_split_splits(int64_t M, int64_t N, int64_t K, int64_t* kchunk);
 omb_status launch_splitk_reduce(int64_t M, int64_t N, int64_t splits, float alpha, float beta,

@@ -29,7 +29,11 @@ constexpr int kGroupM = 8;      // tile rows per L2 group
 constexpr int64_t kSplitMinK = 256;    // least k per split for K > kSplitExactK
 constexpr int64_t kSplitExactK = 512;  // up to this K, splits are single accumulation blocks
 constexpr int64_t kSplitMost = 16;     // most splits considered
-constexpr double kTileRateF32 = 0.33e12;  // FP32 flop/s of one 128x128 workgroup (measured)
+// FP32 flop/s of one 128x128 workgroup, measured: with two per CU (86
+// TFLOP/s over 512 at 8192^3) and alone on its CU (50 TFLOP/s over 256 at
+// 2048^3: one wave per SIMD loses the dual issue of two).
+constexpr double kTileRateF32 = 0.17e12;
+constexpr double kTileRateF32Alone = 0.2e12;
 constexpr double kSplitHbm = 4.0e12;      // bytes/s for the partials' traffic
 constexpr double kSplitLaunch = 4.0e-6;   // the reduction kernel's launch and tail
 
@@ -301,8 +305,8 @@ __global__ void scale_kernel(int64_t M, int64_t N, float beta, float* C, int64_t
 
 }  // namespace
 
-int64_t gemm_k_splits(int64_t M, int64_t N, int64_t K, int64_t align, double tile_rate,
-                      int64_t* kchunk) {
+int64_t gemm_k_splits(int64_t M, int64_t N, int64_t K, int64_t align, double rate_full,
+                      double rate_alone, int64_t* kchunk) {
     *kchunk = K;
     const int64_t tiles = ceil_div(M, 128) * ceil_div(N, 128);
     const int64_t cus = device_cu_count();
@@ -314,12 +318,15 @@ int64_t gemm_k_splits(int64_t M, int64_t N, int64_t K, int64_t align, double til
         *kchunk = align;
         return ceil_div(K, align);
     }
-    // Waves of workgroups times the k each one covers, plus the partials'
-    // write and read and the reduction launch: the split count with the
-    // least modelled time (1 when splitting does not pay).
+    // Rounds of workgroups (two resident per CU) times the k each covers,
+    // at the rate of a workgroup sharing its CU or, when there are no more
+    // workgroups than CUs, alone on it; plus the partials' write and read
+    // and the reduction launch.  The split count with the least modelled
+    // time wins (1 when splitting does not pay).
     auto cost = [&](int64_t splits, int64_t kc) {
-        const double waves = (double)ceil_div(tiles * splits, cus);
-        double t = waves * 2.0 * 128.0 * 128.0 * (double)kc / tile_rate;
+        const int64_t w = tiles * splits;
+        const double work = 2.0 * 128.0 * 128.0 * (double)kc;
+        double t = w <= cus ? work / rate_alone : (double)ceil_div(w, 2 * cus) * work / rate_full;
         if (splits > 1) t += (double)(splits + 1) * (double)(M * N) * 4.0 / kSplitHbm + kSplitLaunch;
         return t;
     };
@@ -341,7 +348,7 @@ int64_t gemm_k_splits(int64_t M, int64_t N, int64_t K, int64_t align, double til
 }
 
 int64_t sgemm_f32_splits(int64_t M, int64_t N, int64_t K, int64_t* kchunk) {
-    return gemm_k_splits(M, N, K, INNER_K, kTileRateF32, kchunk);
+    return gemm_k_splits(M, N, K, INNER_K, kTileRateF32, kTileRateF32Alone, kchunk);
 }
 
 omb_status launch_splitk_reduce(int64_t M, int64_t N, int64_t splits, float alpha, float beta,

@@ -260,12 +260,14 @@ omb_status launch_split_pack(int64_t R, int64_t K, const float* X, int64_t rs, i
     return OMB_OK;
 }
 
-// FP32-equivalent flop/s of one 128x128 workgroup of the 6-product kernel
-// (about 1.2 PFLOP/s of MFMA work over 256 CUs, measured).
-constexpr double kTileRateSplit = 0.78e12;
+// FP32-equivalent flop/s of one 128x128 workgroup of the 6-product kernel,
+// measured: two per CU (190 TFLOP/s over 512 at 8192^3) and alone on its CU
+// (124 TFLOP/s over 256 at 2048^3).
+constexpr double kTileRateSplit = 0.37e12;
+constexpr double kTileRateSplitAlone = 0.48e12;
 
 int64_t sgemm_split_splits(int64_t M, int64_t N, int64_t K, int64_t* kchunk) {
-    return gemm_k_splits(M, N, K, TK * kFoldSlices, kTileRateSplit, kchunk);
+    return gemm_k_splits(M, N, K, TK * kFoldSlices, kTileRateSplit, kTileRateSplitAlone, kchunk);
 }
 
 omb_status launch_sgemm_packed(int64_t M, int64_t N, int64_t K, float alpha, const __bf16* ap,
