@@ -38,15 +38,7 @@ omb_status get_workspace(hipStream_t stream, size_t bytes, Workspace** out) {
     OMB_HIP_TRY(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lock(g_ws_mu);
     auto& slot = ws_map()[WsKey{dev, stream}];
-    if (!slot) {
-        auto ws = std::make_unique<Workspace>();
-        void* t = nullptr;
-        OMB_HIP_TRY(hipMalloc(&t, sizeof(unsigned) * Workspace::kTickets));
-        ws->tickets = static_cast<unsigned*>(t);
-        omb_status st = launch_fill_u32(ws->tickets, Workspace::kTickets, 0u, stream);
-        if (st != OMB_OK) return st;
-        slot = std::move(ws);
-    }
+    if (!slot) slot = std::make_unique<Workspace>();
     Workspace* ws = slot.get();
     if (bytes > ws->bytes) {
         if (ws->buf) OMB_HIP_TRY(hipFree(ws->buf));  // hipFree waits for work in flight

@@ -2,7 +2,7 @@
 // inputs, and a deterministic error norm.  These run beside the FP32 path to
 // state its error; they are not part of any FP32 result.  Also the input
 // generator (uniform[-1,1) from a counter-based hash, bit-identical to the
-// host generator) and a small fill kernel.
+// host generator) and the VALU issue-ceiling calibration kernel.
 #include <algorithm>
 
 #include "common.hpp"
@@ -151,11 +151,6 @@ __global__ __launch_bounds__(NT) void fill_uniform_kernel(float* dst, int64_t n,
         dst[i] = rng_uniform(key, (uint64_t)(i + offset));
 }
 
-__global__ __launch_bounds__(NT) void fill_u32_kernel(unsigned* dst, int64_t n, unsigned v) {
-    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
-        dst[i] = v;
-}
-
 unsigned grid_for(int64_t n, int64_t cap) {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, NT), cap));
 }
@@ -190,13 +185,6 @@ omb_status launch_fill_uniform(float* dst, int64_t n, uint64_t seed, uint64_t st
     if (n <= 0) return OMB_OK;
     hipLaunchKernelGGL(fill_uniform_kernel, dim3(grid_for(n, 16384)), dim3(NT), 0, s, dst, n,
                        rng_key(seed, stream_id), offset);
-    OMB_HIP_TRY(hipGetLastError());
-    return OMB_OK;
-}
-
-omb_status launch_fill_u32(unsigned* dst, int64_t n, unsigned v, hipStream_t s) {
-    if (n <= 0) return OMB_OK;
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(n, 4096)), dim3(NT), 0, s, dst, n, v);
     OMB_HIP_TRY(hipGetLastError());
     return OMB_OK;
 }

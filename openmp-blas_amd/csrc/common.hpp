@@ -39,8 +39,6 @@ __host__ __device__ inline int64_t round_up(int64_t a, int64_t b) { return ceil_
 struct Workspace {
     void* buf = nullptr;
     size_t bytes = 0;
-    unsigned* tickets = nullptr;  // kTickets counters, zero between calls
-    static constexpr int kTickets = 1 << 16;
 };
 
 // Compute units of the current device (cached per device).
@@ -117,7 +115,6 @@ omb_status launch_sdot_ws(int64_t n, const float* x, const float* y, float* resu
 
 omb_status launch_fill_uniform(float* dst, int64_t n, uint64_t seed, uint64_t stream_id,
                                int64_t offset, hipStream_t s);
-omb_status launch_fill_u32(unsigned* dst, int64_t n, unsigned v, hipStream_t s);
 
 // ----------------------------------------------------- device helpers -----
 #if defined(__HIPCC__)

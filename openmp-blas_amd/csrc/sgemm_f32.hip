@@ -29,6 +29,7 @@ constexpr int kGroupM = 8;      // tile rows per L2 group
 constexpr int64_t kSplitMinK = 256;    // least k per split for K > kSplitExactK
 constexpr int64_t kSplitExactK = 512;  // up to this K, splits are single accumulation blocks
 constexpr int64_t kSplitMost = 16;     // most splits considered
+constexpr double kSplitMaxBytes = 128.0 * (1 << 20);  // most partial-sum bytes for K > 512
 // FP32 flop/s of one 128x128 workgroup, measured: with two per CU (86
 // TFLOP/s over 512 at 8192^3) and alone on its CU (50 TFLOP/s over 256 at
 // 2048^3: one wave per SIMD loses the dual issue of two).
@@ -336,6 +337,7 @@ int64_t gemm_k_splits(int64_t M, int64_t N, int64_t K, int64_t align, double rat
     for (int64_t sp = 2; sp <= most; ++sp) {
         const int64_t kc = round_up(ceil_div(K, sp), align);
         const int64_t n = ceil_div(K, kc);
+        if ((double)n * (double)(M * N) * 4.0 > kSplitMaxBytes) break;  // partials too large
         const double t = cost(n, kc);
         if (n > 1 && t < 0.95 * best_t) {
             best = n;
