@@ -9,6 +9,8 @@
 // Host buffers exercise the staging path, device buffers the pass-through.
 //
 // Usage: test_dropin [--quick]   (exit status 0 = all passed)
+//        test_dropin --time-host   (host-buffer rates, PCIe staging included)
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -444,8 +446,66 @@ void test_errors() {
 
 }  // namespace
 
+// Wall time of the drop-in helpers on host buffers: each call stages its
+// operands to the device and the result back, as a caller of the reference
+// helpers with host memory would see it.  Mean of 3 calls after one.
+template <class F>
+double host_call_seconds(F&& f) {
+    f();
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < 3; ++i) f();
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / 3;
+}
+
+int time_host() {
+    namespace lay = amt::layout;
+    const std::size_t n = 8192;
+    std::vector<float> a(n * n), b(n * n), c(n * n);
+    for (std::size_t i = 0; i < n * n; ++i) {
+        a[i] = (float)(std::rand() % 100) / 100.f;
+        b[i] = (float)(std::rand() % 100) / 100.f;
+    }
+    const std::size_t na[2] = {n, n}, wrow[2] = {n, 1};
+    for (int mode : {OMB_MODE_PLAIN_F32, OMB_MODE_SPLIT_BF16}) {
+        amt::dropin::settings().gemm_mode = mode;
+        const double t = host_call_seconds([&] {
+            amt::mtm_helper(c.data(), na, wrow, a.data(), na, wrow, b.data(), na, wrow,
+                            lay::last_order{});
+        });
+        std::printf("host mtm 8192^3 %s: %.3f s, %.2f TFLOP/s (768 MiB to the device, 256 MiB back)\n",
+                    mode == OMB_MODE_PLAIN_F32 ? "plain" : "split-bf16", t,
+                    2.0 * n * n * n / t / 1e12);
+    }
+    amt::dropin::settings().gemm_mode = OMB_MODE_PLAIN_F32;
+    const std::size_t m = 16384;
+    std::vector<float> ga(m * m, 0.5f), x(m, 0.25f), y(m, 0.f);
+    const std::size_t nga[2] = {m, m}, wga[2] = {m, 1}, nx[2] = {m, 1}, ny[2] = {m, 1};
+    const double tv = host_call_seconds([&] {
+        amt::mtv_helper(y.data(), ny, ga.data(), nga, wga, x.data(), nx, 0, lay::last_order{});
+    });
+    std::printf("host mtv 16384^2 row-major: %.4f s, %.1f GB/s of A\n", tv, 4.0 * m * m / tv / 1e9);
+    const std::size_t nd = std::size_t(1) << 28;
+    std::vector<float> dx(nd, 0.5f), dy(nd, 0.25f);
+    float r = 0.f;
+    const std::size_t ndx[2] = {nd, 1};
+    const double td = host_call_seconds(
+        [&] { amt::dot_prod_helper(&r, dx.data(), ndx, dy.data(), ndx, 0); });
+    std::printf("host dot 2^28: %.4f s, %.1f GB/s of x and y\n", td, 8.0 * nd / td / 1e9);
+    std::fflush(stdout);
+    return 0;
+}
+
 int main(int argc, char** argv) {
     const bool quick = argc > 1 && std::strcmp(argv[1], "--quick") == 0;
+    if (argc > 1 && std::strcmp(argv[1], "--time-host") == 0) {
+        try {
+            amt::dropin::check(omb_init(), "omb_init");
+            return time_host();
+        } catch (const std::exception& e) {
+            std::printf("FAIL: exception: %s\n", e.what());
+            return 2;
+        }
+    }
     std::srand(0);
     try {
         amt::dropin::check(omb_init(), "omb_init");
