@@ -203,6 +203,20 @@ void dot_prod_helper(ValueType* c, ValueType const* a, SizeType const* na, Value
     }
 }
 
+// The reference's tuning entry (include/dot.hpp:59-89): the same *c = a . b
+// with caller-chosen cache block sizes for the CPU loops.  The GPU kernel's
+// grid depends only on n, so the blocks are accepted and not used.
+template <class ValueType, class SizeType>
+void dot_prod_helper_tuning_param(ValueType* c, ValueType const* a, ValueType const* b,
+                                  SizeType n, int max_threads, SizeType const block1,
+                                  SizeType const block2, SizeType const block3) {
+    (void)block1;
+    (void)block2;
+    (void)block3;
+    const SizeType na[2] = {n, 1};
+    dot_prod_helper(c, a, na, b, na, max_threads);
+}
+
 // Tags of transpose_helper (the reference's amt::tag, include/utils.hpp:71-73).
 namespace tag {
 struct inplace {};
@@ -399,6 +413,25 @@ auto dot_prod(TC& c, const TA& a, const TB& b,
     const auto* ap = a.data();
     const auto* bp = b.data();
     return [=] { dot_prod_helper(cp, ap, na.data(), bp, nb.data(), 1); };
+}
+
+// include/dot.hpp:91-140: c is a scalar of the vectors' value type,
+// captured by reference as in the reference.
+template <class Out, class TA, class TB>
+auto dot_prod_tuning_param(Out& c, const TA& a, const TB& b, std::size_t b1, std::size_t b2,
+                           std::size_t b3, std::optional<std::size_t> num_threads) {
+    (void)num_threads;
+    const auto na = dropin::ext2(a), nb = dropin::ext2(b);
+    if (!(dropin::is_vec(na) && dropin::is_vec(nb)))
+        throw std::runtime_error("amt::dot_prod: both tensor must be vector");
+    if (na[0] * na[1] != nb[0] * nb[1])
+        throw std::runtime_error("amt::dot_prod: both vector must be of same size");
+    const auto* ap = a.data();
+    const auto* bp = b.data();
+    const std::size_t n = na[0] * na[1];
+    return [&c, ap, bp, n, b1, b2, b3] {
+        dot_prod_helper_tuning_param(&c, ap, bp, n, 1, b1, b2, b3);
+    };
 }
 
 template <class TC, class TA, class TB>

@@ -369,6 +369,15 @@ void test_tensor_level() {
         int64_t dd = 0;
         for (float t : x) dd += static_cast<int64_t>(t) * static_cast<int64_t>(t);
         CHECK(static_cast<int64_t>(r) == dd, "tensor dot %zu", m);
+        // the tuning entries of src/dot_tuning.cpp: blocks accepted, same result
+        float rt = -1.f;
+        amt::dot_prod_tuning_param(rt, CV::last_order(x.data(), 1, k),
+                                   CV::last_order(x.data(), 1, k), 1024, 4096, 65536, 8)();
+        CHECK(rt == r, "tensor dot tuning %zu", m);
+        float rh = -1.f;
+        amt::dot_prod_helper_tuning_param(&rh, x.data(), x.data(), k, 8, std::size_t(64),
+                                          std::size_t(128), std::size_t(256));
+        CHECK(rh == r, "dot helper tuning %zu", m);
         // outer product into both orders, transpose out of and in place
         for (int first = 0; first < 2; ++first) {
             Mat O(m, k, first);
