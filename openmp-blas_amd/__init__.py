@@ -314,6 +314,13 @@ def dot_prod(c, a, b, num_threads=None, stream=None):
     return run
 
 
+def dot_prod_tuning_param(c, a, b, b1, b2, b3, num_threads=None, stream=None):
+    """amt::dot_prod_tuning_param (include/dot.hpp:91-140): dot_prod with the
+    CPU cache block sizes b1, b2, b3, which the GPU kernel does not use (its
+    grid depends only on the length)."""
+    return dot_prod(c, a, b, num_threads, stream)
+
+
 def outer_prod(c, a, b, num_threads=None, stream=None):
     """amt::outer_prod (include/outer.hpp:51-122): C += a b^T for vectors a
     (length M) and b (length N) and an M x N matrix C of either layout."""

@@ -127,6 +127,10 @@ def test_dot_closed_form(dev, ompblas):
             assert got == exact, s
         else:
             assert abs(got - exact) <= (s * u / (1 - s * u)) * exact, s
+        # src/dot_tuning.cpp's entry: block sizes accepted, same bits
+        rt = torch.full((1,), -1.0, device=dev)
+        ompblas.dot_prod_tuning_param(rt, a, a, 1024, 4096, 1 << 16)()
+        assert torch.equal(rt, r), s
 
 
 def test_dot_config2_error_vs_reference(dev, ompblas):
