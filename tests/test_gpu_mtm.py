@@ -321,3 +321,56 @@ def test_packed_split_path_matches_one_call(dev, ompblas, shape, nprod):
                                            got.data_ptr(), got.stride(0), got.stride(1), nprod,
                                            None))
         assert torch.equal(got, want), (shape, nprod, c_order)
+
+
+def test_concurrent_streams_equal_sequential(dev, ompblas):
+    """Calls on different streams use their own workspaces (split-BF16 packed
+    operands, split-K partials, column-GEMV and dot partials), so four
+    streams running at once give the same bits as the same calls in sequence
+    on the default stream."""
+    jobs = []
+    for s in range(4):
+        m, n, k = 96 + 32 * s, 160 - 16 * s, 1024 + 512 * s       # few tiles: K is split
+        a = torch.empty(m, k, device=dev)
+        b = torch.empty(k, n, device=dev)
+        ompblas.fill_uniform(a, 20 + s, 0)
+        ompblas.fill_uniform(b, 20 + s, 1)
+        v = torch.empty(3000 + 100 * s, 700, device=dev).t()   # column-major 700 x (3000 + 100s)
+        ompblas.fill_uniform(v, 20 + s, 2)
+        x = torch.empty(v.shape[1], device=dev)
+        ompblas.fill_uniform(x, 20 + s, 3)
+        jobs.append((m, n, k, a, b, v, x))
+    torch.cuda.synchronize()
+
+    def outputs(job):  # zeroed before any call is issued
+        m, n, k, a, b, v, x = job
+        return [torch.zeros(m, n, device=dev), torch.zeros(m, n, device=dev),
+                torch.zeros(v.shape[0], device=dev), torch.zeros(1, device=dev)]
+
+    def run(job, outs, stream):
+        m, n, k, a, b, v, x = job
+        sp = stream.cuda_stream if stream is not None else None
+        for mode, c in zip((1, 0), outs[:2]):
+            ompblas.check(ompblas.sgemm(m, n, k, 1.0, a.data_ptr(), k, 1, b.data_ptr(), n, 1, 0.0,
+                                        c.data_ptr(), n, 1, mode, stream=sp))
+        ompblas.check(ompblas.sgemv(v.shape[0], v.shape[1], 1.0, v.data_ptr(), v.stride(0),
+                                    v.stride(1), x.data_ptr(), 1.0, outs[2].data_ptr(), stream=sp))
+        ompblas.check(ompblas.sdot(x.numel(), x.data_ptr(), x.data_ptr(), outs[3].data_ptr(),
+                                   stream=sp))
+        return outs
+
+    want = [outputs(j) for j in jobs]
+    torch.cuda.synchronize()
+    for j, o in zip(jobs, want):
+        run(j, o, None)
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream() for _ in jobs]
+    for rep in range(2):
+        got = [outputs(j) for j in jobs]
+        torch.cuda.synchronize()
+        for j, o, st in zip(jobs, got, streams):
+            run(j, o, st)
+        torch.cuda.synchronize()
+        for w, g in zip(want, got):
+            for i, (p, q) in enumerate(zip(w, g)):
+                assert torch.equal(p, q), (rep, i)
