@@ -238,7 +238,7 @@ def main():
 
     # --------------------------------------------------------- GEMV / dot --
     del C
-    gemv, dot = {}, {}
+    gemv, dot, refs = {}, {}, {}
     if "gemv" in legs:
         log("sgemv 16384^2 (seed 1, row and column major)")
         m = 16384
@@ -259,6 +259,30 @@ def main():
                             "frac_of_hbm_peak": nbytes / t_g / PEAK_HBM, "bytes": nbytes}
         kernels["sgemv_row_kernel (16384^2)"] = gemv["row"]
         kernels["sgemv_col_kernel (16384^2)"] = gemv["col"]
+        # errors against an FP64 product (GPU check kernel), y = A x from y = 0
+        with torch.cuda.stream(stream):
+            y64 = torch.empty(m, dtype=torch.float64, device=dev)
+        pkg.check(lib.omb_check_dgemv(m, m, GA.data_ptr(), m, 1, gx.data_ptr(), None,
+                                      y64.data_ptr(), sp))
+        sync()
+        refs["y64"] = y64.cpu().numpy()
+        for layout, (rs, cs) in (("row", (m, 1)), ("col", (1, m))):
+            gy.zero_()
+            sync()
+            pkg.check(lib.omb_sgemv(m, m, 1.0, GA.data_ptr(), rs, cs, gx.data_ptr(), 0.0,
+                                    gy.data_ptr(), sp))
+            sync()
+            # the column-major call reads A as its transpose: compare with A^T x
+            want = refs["y64"] if layout == "row" else None
+            if want is None:
+                pkg.check(lib.omb_check_dgemv(m, m, GA.data_ptr(), 1, m, gx.data_ptr(), None,
+                                              y64.data_ptr(), sp))
+                sync()
+                want = y64.cpu().numpy()
+                refs["y64_col"] = want
+            got = gy.cpu().numpy().astype(np.float64)
+            gemv[layout]["max_rel"] = float(np.abs(got - want).max() / np.abs(want).max())
+            gemv[layout]["frob_rel"] = float(np.linalg.norm(got - want) / np.linalg.norm(want))
         del GA
     if "dot" in legs:
         log("sdot 2^28 (seed 1)")
@@ -276,6 +300,18 @@ def main():
         dot = {"us": t_d * 1e6, "gbps": 8.0 * nd / t_d / 1e9,
                "frac_of_hbm_peak": 8.0 * nd / t_d / PEAK_HBM}
         kernels["sdot_kernel (2^28)"] = dot
+        # |s - s64| / |s64|, and |s - s64| / sum |x_i y_i| (independent of
+        # cancellation; the classic gamma_n bound is vacuous here, n u = 16)
+        with torch.cuda.stream(stream):
+            r64 = torch.empty(1, dtype=torch.float64, device=dev)
+            sabs = (dx.double().abs_().mul_(dy.double().abs_())).sum()
+        pkg.check(lib.omb_check_ddot(nd, dx.data_ptr(), dy.data_ptr(), r64.data_ptr(), sp))
+        sync()
+        s64, sa = float(r64.item()), float(sabs.item())
+        refs["dot64"], refs["dot_sum_abs"] = s64, sa
+        err = abs(float(dr.item()) - s64)
+        dot["rel_err"] = err / abs(s64)
+        dot["err_over_sum_abs"] = err / sa
         del dx, dy
     sync()
 
@@ -335,7 +371,7 @@ def main():
     cpu, cpu_ref = None, None
     if "cpu" in legs:
         cpu = cpu_baseline(args.cpu_budget)
-        cpu_ref = cpu_reference_legs(pkg, torch, dev, stream)
+        cpu_ref = cpu_reference_legs(pkg, torch, dev, stream, refs)
 
     res = {
         "metric": METRIC,
@@ -701,7 +737,7 @@ def c1_leg(pkg, lib, torch, dev, sp, timer):
     return out
 
 
-def cpu_reference_legs(pkg, torch, dev, stream):
+def cpu_reference_legs(pkg, torch, dev, stream, refs=None):
     """The oracle (the reference algorithm, OpenMP on all host cores) on the
     bandwidth-bound configurations at full size: SGEMV 16384^2 in both
     layouts (MaxIter 10, src/mtv.cpp:332) and SDOT 2^28 (MaxIter 100,
@@ -727,10 +763,23 @@ def cpu_reference_legs(pkg, torch, dev, stream):
     del Ad
     y = np.zeros(n, np.float32)
     nbytes = 4.0 * (n * n + 2 * n)
+    refs = refs or {}
+
+    def errs(yv, want):
+        d = yv.astype(np.float64) - want
+        return {"max_rel": float(np.abs(d).max() / np.abs(want).max()),
+                "frob_rel": float(np.linalg.norm(d) / np.linalg.norm(want))}
+
     t = _mean_time(lambda: helpers.oracle_mtv(y, a, x, blk, cores), 10)
     out["mtv_row_major_16384"] = {"ms": t * 1e3, "gbps": nbytes / t / 1e9}
+    if "y64" in refs:
+        out["mtv_row_major_16384"].update(errs(y, refs["y64"]))      # y = A x overwrites
     t = _mean_time(lambda: helpers.oracle_mtv(y, a.T, x, blk, cores), 10)   # column-major view
     out["mtv_col_major_16384"] = {"ms": t * 1e3, "gbps": (nbytes + 4 * n) / t / 1e9}
+    if "y64_col" in refs:
+        y[:] = 0
+        helpers.oracle_mtv(y, a.T, x, blk, cores)                     # y += A^T x from 0
+        out["mtv_col_major_16384"].update(errs(y, refs["y64_col"]))
     del a
     nd = 1 << 28
     with torch.cuda.stream(stream):
@@ -743,6 +792,10 @@ def cpu_reference_legs(pkg, torch, dev, stream):
     del xd, yd
     t = _mean_time(lambda: helpers.oracle_dot(xh, yh, blk, cores), 100)
     out["dot_2^28"] = {"ms": t * 1e3, "gbps": 8.0 * nd / t / 1e9}
+    if "dot64" in refs:
+        err = abs(helpers.oracle_dot(xh, yh, blk, cores) - refs["dot64"])
+        out["dot_2^28"]["rel_err"] = err / abs(refs["dot64"])
+        out["dot_2^28"]["err_over_sum_abs"] = err / refs["dot_sum_abs"]
     return out
 
 
