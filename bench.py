@@ -96,6 +96,12 @@ def main():
               "(the multi-GPU sharding is a design, not a measured path)", file=sys.stderr)
         sys.exit(2)
 
+    # The oracle's OpenMP team: one thread per core of the process, packed
+    # (read by the OpenMP runtime when it starts, so before any import that
+    # could start it).
+    os.environ.setdefault("OMP_PLACES", "cores")
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+
     legs = {"plain", "errors", "gemv", "dot", "level2", "grouped", "serving", "c1", "cpu"}
     if args.legs != "all":
         legs = {x.strip() for x in args.legs.split(",") if x.strip()}
@@ -839,6 +845,8 @@ def cpu_baseline(budget_s: float):
             "sample": f"full 8192^3 FP32 mtm (C += A*B), mean of {iters} timed calls after "
                       f"1 warm-up ({t_first:.2f} s)",
             "median_s": float(np.median(times)), "mean_s": mean, "cpu_model": model,
+            "omp_places": os.environ.get("OMP_PLACES"),
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
             "blocking": blk.as_dict(), "machine": platform.machine()}
 
 
