@@ -34,8 +34,7 @@ __host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + 
 __host__ __device__ inline int64_t round_up(int64_t a, int64_t b) { return ceil_div(a, b) * b; }
 
 // --------------------------------------------------------------- workspace --
-// Per-stream scratch memory: one grow-only device buffer plus a block of
-// ticket counters that kernels leave at zero when they finish.
+// Per-stream scratch memory: one grow-only device buffer.
 struct Workspace {
     void* buf = nullptr;
     size_t bytes = 0;

@@ -3,6 +3,7 @@ package, the counter-based input generator in numpy, and error metrics."""
 from __future__ import annotations
 
 import ctypes
+import json
 import os
 import sys
 
@@ -15,6 +16,15 @@ if ROOT not in sys.path:
 import __graft_entry__ as graft  # noqa: E402
 
 ORACLE_LIB = os.path.join(ROOT, "oracle", "lib", "liboracle.so")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def golden(name: str) -> dict:
+    """A known-answer fixture from tests/golden (JSON data transcribed from
+    the reference's tests and report; each file cites its source lines)."""
+    with open(os.path.join(GOLDEN, name + ".json")) as f:
+        return json.load(f)
+
 
 _VP = ctypes.c_void_p
 _I64 = ctypes.c_int64
@@ -38,7 +48,7 @@ class Blocking(ctypes.Structure):
 
 
 # Cache geometry of the reference report's laptop (doc/intro.tex:182-210).
-DOC_LAPTOP = ((32768, 8, 64), (262144, 4, 64), (16777216, 16, 64))
+DOC_LAPTOP = tuple(tuple(lv) for lv in golden("blocking_doc_laptop")["levels"])
 
 
 def ensure_built():

@@ -8,21 +8,13 @@ from helpers import DOC_LAPTOP, blocking, oracle_dot, oracle_mtm, oracle_mtv, or
 
 
 # ----------------------------------------------------------- blocking ----
-def test_blocking_doc_tables_float():
-    """doc/matrix_times_matrix.tex:290-302 (MR/NR), :378-388 (KB), :440-450 (MB),
-    :487-497 (NB); doc/inner_product.tex:258-268,316-326,370-380 (dot blocks),
-    all for the report's laptop (doc/intro.tex:182-210)."""
-    b = blocking(DOC_LAPTOP, 4, 0).as_dict()
-    assert (b["mr"], b["nr"]) == (16, 6)
-    assert (b["kc"], b["mc"], b["nc"]) == (384, 32, 4776)
-    assert (b["dot_sec1"], b["dot_sec2"], b["dot_sec3"]) == (16384, 4096, 32768)
-
-
-def test_blocking_doc_tables_double():
-    b = blocking(DOC_LAPTOP, 8, 0).as_dict()
-    assert (b["mr"], b["nr"]) == (8, 6)
-    assert (b["kc"], b["mc"], b["nc"]) == (256, 32, 3582)
-    assert (b["dot_sec1"], b["dot_sec2"], b["dot_sec3"]) == (8192, 2048, 16384)
+@pytest.mark.parametrize("elem", [4, 8])
+def test_blocking_doc_tables(elem):
+    """The report's tuning tables for its laptop, float and double
+    (tests/golden/blocking_doc_laptop.json cites the doc lines)."""
+    want = helpers.golden("blocking_doc_laptop")["tables"][str(elem)]
+    got = blocking(DOC_LAPTOP, elem, 0).as_dict()
+    assert {k: got[k] for k in want} == want
 
 
 def test_ct_sqrt_quirk():
@@ -43,35 +35,24 @@ def test_blocking_linux_l3_assoc_slot():
 
 
 # ---------------------------------------------------------------- pack ----
-def _iota33():
-    # uBLAS first_order 3x3 tensor holding 1..9 in storage order.
-    return np.arange(1, 10, dtype=np.float32)
-
-
-def test_pack_known_answer():
-    """test/test.pack.cpp:8-44."""
+@pytest.mark.parametrize("which", ["pack", "pack_trans"])
+def test_pack_known_answer(which):
+    """test/test.pack.cpp:8-44 (pack) and :46-83 (transposed pack): row panels
+    of MR = 2 over a 3x3 column-major tensor of 1..9."""
+    g = helpers.golden("pack_3x3")
     o = helpers.oracle()
-    inp = _iota33()
-    out = _iota33().copy()
-    w = (helpers.ctypes.c_int64 * 2)(1, 3)
-    for i in range(0, 3, 2):
-        ib = min(2, 3 - i)
-        o.oracle_pack_f32(out[i * 3:].ctypes.data, ib, inp[i * 1:].ctypes.data, w, ib, 3)
-    for i_in, i_out in [(0, 0), (1, 1), (2, 6), (3, 2), (4, 3), (5, 7), (6, 4), (7, 5), (8, 8)]:
+    fn = o.oracle_pack_f32 if which == "pack" else o.oracle_pack_trans_f32
+    rows, cols = g["extents"]
+    inp = np.array(g["input"], dtype=np.float32)
+    out = np.array(g["out_init"], dtype=np.float32)
+    w = (helpers.ctypes.c_int64 * 2)(*g["strides"])
+    step = g["strides"][0] if which == "pack" else g["strides"][1]
+    for i in range(0, rows, g["mr"]):
+        ib = min(g["mr"], rows - i)
+        fn(out[i * cols:].ctypes.data, ib, inp[i * step:].ctypes.data, w, ib, cols)
+    for i_in, i_out in g[which]["pairs"]:
         assert inp[i_in] == out[i_out]
-
-
-def test_pack_trans_known_answer():
-    """test/test.pack.cpp:46-83."""
-    o = helpers.oracle()
-    inp = _iota33()
-    out = _iota33().copy()
-    w = (helpers.ctypes.c_int64 * 2)(1, 3)
-    for i in range(0, 3, 2):
-        ib = min(2, 3 - i)
-        o.oracle_pack_trans_f32(out[i * 3:].ctypes.data, ib, inp[i * 3:].ctypes.data, w, ib, 3)
-    for i_in, i_out in [(0, 0), (1, 2), (2, 4), (3, 1), (4, 3), (5, 5), (6, 6), (7, 7), (8, 8)]:
-        assert inp[i_in] == out[i_out]
+    assert out.tolist() == g[which]["out"]
 
 
 # ----------------------------------------------------------------- mtm ----
