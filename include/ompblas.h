@@ -53,7 +53,9 @@ const char* omb_version(void);
  * C = alpha*A*B + beta*C;  A: M x K, B: K x N, C: M x N.
  * Replaces amt::mtm_helper (include/mtm.hpp:116-122), which computes
  * C += A*B (alpha = beta = 1) for any first/last-order mix of A, B, C.
- * beta == 0 does not read C. */
+ * beta == 0 does not read C.  C must share no element with A or B (A and B
+ * may alias each other); as in BLAS this is not checked, since disjoint
+ * submatrix views of one buffer are legitimate and their spans overlap. */
 omb_status omb_sgemm(int64_t M, int64_t N, int64_t K, float alpha,
                      const float* A, int64_t rsA, int64_t csA,
                      const float* B, int64_t rsB, int64_t csB,
@@ -80,7 +82,8 @@ omb_status omb_sgemm_packed(int64_t M, int64_t N, int64_t K, float alpha,
  * Replaces amt::mtv_helper first_order (include/mtv.hpp:15-22; y += A*x,
  * i.e. beta = 1) and last_order (include/mtv.hpp:72-79; y = A*x, beta = 0).
  * vtm (include/mtv.hpp:164-236) is this call on the transposed view
- * (swap M/N and rs/cs). */
+ * (swap M/N and rs/cs).  y must share no element with A or x (not
+ * checked). */
 omb_status omb_sgemv(int64_t M, int64_t N, float alpha,
                      const float* A, int64_t rsA, int64_t csA,
                      const float* x, float beta, float* y, omb_stream_t stream);

@@ -33,6 +33,9 @@ std::map<WsKey, std::unique_ptr<Workspace>>& ws_map() {
 void set_error(const std::string& msg) { t_error = msg; }
 void clear_error() { t_error.clear(); }
 
+// The returned buffer stays valid until the next call for the same (device,
+// stream) asks for more; calls on one stream are issued by one host thread
+// at a time, so no call still holds it then.
 omb_status get_workspace(hipStream_t stream, size_t bytes, Workspace** out) {
     int dev = 0;
     OMB_HIP_TRY(hipGetDevice(&dev));
