@@ -62,15 +62,17 @@ omb_status resolve_layout(const char* name, int64_t rows, int64_t cols, int64_t 
 // Plain FP32 SGEMM, row-major C (ldc), A K-contiguous iff a_kc, B N-contiguous
 // iff b_nc.  C = alpha*A*B + beta*C.
 // K splits for an M x N x K GEMM of 128 x 128 tiles (1: none) and the k per
-// split, a multiple of `align` (the kernel's accumulation block).  Up to
-// K = 512, and only when the tiles leave CUs idle, each split is one block,
-// which gives the unsplit kernel's bits.  Beyond, at least 256 k per split,
-// and the count that minimises a model of rounds x k per workgroup (at
-// `rate_full` flop/s per workgroup with two per CU, `rate_alone` with one)
-// plus the partials' traffic: deterministic, with a different rounding.  A
-// launch with more than one split needs splits * M * N floats of partial
-// sums, from `part` when given, else from the stream's workspace;
-// launch_splitk_reduce adds them in split order.
+// split, a multiple of `align` (the kernel's accumulation block), from a
+// model of rounds x k per workgroup (at `rate_full` flop/s per workgroup
+// with two per CU, `rate_alone` with one) plus the partials' traffic.  The
+// first choice is exact: one block per split, whose in-order sum is the
+// unsplit kernel's own fold, so the bits are the unsplit ones on any device
+// (taken when the partials fit in 128 MiB and the model finds it 5%
+// faster than no split).  For K > 512 a coarser split (at least 256 k each,
+// deterministic, with a different rounding) replaces it only when it is
+// more than 10% faster.  A launch with more than one split needs
+// splits * M * N floats of partial sums, from `part` when given, else from
+// the stream's workspace; launch_splitk_reduce adds them in split order.
 int64_t gemm_k_splits(int64_t M, int64_t N, int64_t K, int64_t align, double rate_full,
                       double rate_alone, int64_t* kchunk);
 int64_t sgemm_f32_splits(int64_t M, int64_t N, int64_t K, int64_t* kchunk);

@@ -26,10 +26,10 @@ constexpr int NT = 256;         // threads per workgroup
 constexpr int INNER_K = 128;    // k steps per inner accumulation block
 constexpr int PAD = 4;          // LDS row padding (floats), keeps float4 alignment
 constexpr int kGroupM = 8;      // tile rows per L2 group
-constexpr int64_t kSplitMinK = 256;    // least k per split for K > kSplitExactK
-constexpr int64_t kSplitExactK = 512;  // up to this K, splits are single accumulation blocks
-constexpr int64_t kSplitMost = 16;     // most splits considered
-constexpr double kSplitMaxBytes = 128.0 * (1 << 20);  // most partial-sum bytes for K > 512
+constexpr int64_t kSplitMinK = 256;    // least k per coarse split (K > kSplitExactK)
+constexpr int64_t kSplitExactK = 512;  // up to this K, splits are only single accumulation blocks
+constexpr int64_t kSplitMost = 16;     // most coarse splits considered
+constexpr double kSplitMaxBytes = 128.0 * (1 << 20);  // most partial-sum bytes
 // FP32 flop/s of one 128x128 workgroup, measured: with two per CU (86
 // TFLOP/s over 512 at 8192^3) and alone on its CU (50 TFLOP/s over 256 at
 // 2048^3: one wave per SIMD loses the dual issue of two).
@@ -37,6 +37,7 @@ constexpr double kTileRateF32 = 0.17e12;
 constexpr double kTileRateF32Alone = 0.2e12;
 constexpr double kSplitHbm = 4.0e12;      // bytes/s for the partials' traffic
 constexpr double kSplitLaunch = 4.0e-6;   // the reduction kernel's launch and tail
+constexpr double kSplitReduceStep = 1.0e-6;  // one round trip per 8 partials of an element
 
 struct Args {
     int64_t M, N, K;
@@ -277,7 +278,17 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(int64_t M, int64_t N
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
          i += (int64_t)gridDim.x * blockDim.x) {
         float s = part[i];
-        for (int64_t t = 1; t < splits; ++t) s += part[t * total + i];
+        int64_t t = 1;
+        // Eight loads in flight per step (a long split list is otherwise
+        // one memory round trip per partial); the adds stay in split order.
+        for (; t + 8 <= splits; t += 8) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = part[(t + u) * total + i];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s += v[u];
+        }
+        for (; t < splits; ++t) s += part[t * total + i];
         const int64_t m = i / N, n = i - m * N;
         float* c = C + m * ldc + n;
         const float v = alpha * s;
@@ -312,34 +323,47 @@ int64_t gemm_k_splits(int64_t M, int64_t N, int64_t K, int64_t align, double rat
     const int64_t tiles = ceil_div(M, 128) * ceil_div(N, 128);
     const int64_t cus = device_cu_count();
     if (tiles <= 0 || K <= align) return 1;
-    if (K <= kSplitExactK) {
-        if (tiles >= cus) return 1;
-        // One accumulation block per split: the in-order sum of the
-        // partials is the unsplit kernel's own fold, so the bits are equal.
-        *kchunk = align;
-        return ceil_div(K, align);
-    }
     // Rounds of workgroups (two resident per CU) times the k each covers,
     // at the rate of a workgroup sharing its CU or, when there are no more
-    // workgroups than CUs, alone on it; plus the partials' write and read
-    // and the reduction launch.  The split count with the least modelled
-    // time wins (1 when splitting does not pay).
+    // workgroups than CUs, alone on it; plus the partials' write and read,
+    // the reduction launch and its round trips.
     auto cost = [&](int64_t splits, int64_t kc) {
         const int64_t w = tiles * splits;
         const double work = 2.0 * 128.0 * 128.0 * (double)kc;
         double t = w <= cus ? work / rate_alone : (double)ceil_div(w, 2 * cus) * work / rate_full;
-        if (splits > 1) t += (double)(splits + 1) * (double)(M * N) * 4.0 / kSplitHbm + kSplitLaunch;
+        if (splits > 1)
+            t += (double)(splits + 1) * (double)(M * N) * 4.0 / kSplitHbm + kSplitLaunch +
+                 (double)ceil_div(splits - 1, 8) * kSplitReduceStep;
         return t;
     };
+    const double part_bytes = (double)(M * N) * 4.0;
+    // Exact choices first: no split, or one accumulation block per split.
+    // The in-order sum of single blocks is the unsplit kernel's own fold
+    // (outer = ((b0 + b1) + b2) ...), so either gives the unsplit bits, on
+    // any device.  The split is taken when the model finds it 5% faster.
     int64_t best = 1;
     double best_t = cost(1, K);
+    const int64_t nblk = ceil_div(K, align);
+    if ((double)nblk * part_bytes <= kSplitMaxBytes) {
+        const double t = cost(nblk, align);
+        if (t < 0.95 * best_t) {
+            best = nblk;
+            best_t = t;
+            *kchunk = align;
+        }
+    }
+    if (K <= kSplitExactK) return best;
+    // Coarser splits (at least 256 k each) round differently: one is taken
+    // only when it is more than 10% faster than the exact choice.
+    const double exact_t = best_t;
     const int64_t most = std::min<int64_t>(kSplitMost, K / kSplitMinK);
     for (int64_t sp = 2; sp <= most; ++sp) {
         const int64_t kc = round_up(ceil_div(K, sp), align);
         const int64_t n = ceil_div(K, kc);
-        if ((double)n * (double)(M * N) * 4.0 > kSplitMaxBytes) break;  // partials too large
+        if (kc == align) continue;  // the exact choice above
+        if ((double)n * part_bytes > kSplitMaxBytes) break;  // partials too large
         const double t = cost(n, kc);
-        if (n > 1 && t < 0.95 * best_t) {
+        if (n > 1 && t < 0.9 * exact_t && t < best_t) {
             best = n;
             best_t = t;
             *kchunk = kc;

@@ -123,14 +123,16 @@ def test_split_k_integer_exact(dev, ompblas, combo, mode):
 
 
 @pytest.mark.parametrize("mode", MODES)
-def test_split_k_up_to_512_gives_the_unsplit_bits(dev, ompblas, mode):
-    """Up to K = 512 a split product splits K into single accumulation
-    blocks, whose in-order sum is the unsplit kernel's own fold: its rows
-    equal, bit for bit, the same rows of a taller product (enough tiles that
-    it does not split)."""
-    N = 300
-    big_m = 128 * 100                      # 100 x 3 tiles: more than the CUs, no split
-    for m, k in [(200, 500), (1, 300), (129, 512), (77, 129)]:
+def test_exact_k_splits_give_the_unsplit_bits(dev, ompblas, mode):
+    """A product with few tiles splits K into single accumulation blocks
+    whenever their partials fit (for any K here), and their in-order sum is
+    the unsplit kernel's own fold: its rows equal, bit for bit, the same rows
+    of a taller product that does not split (512 tiles: two workgroups per
+    CU, one round, so a split cannot pay)."""
+    N = 512
+    big_m = 128 * 128                      # 128 x 4 tiles
+    for m, k in [(200, 500), (1, 300), (129, 512), (77, 129), (200, 3000), (1, 4100),
+                 (129, 1024), (77, 2000)]:
         a_big = torch.empty(big_m, k, device=dev)
         ompblas.fill_uniform(a_big, 11, 0)
         b = torch.empty(k, N, device=dev)
@@ -142,11 +144,12 @@ def test_split_k_up_to_512_gives_the_unsplit_bits(dev, ompblas, mode):
         assert torch.equal(c, c_big[:m]), (mode, m, k)
 
 
-def test_split_k_deterministic_and_within_error_bound(dev, ompblas):
-    """A split-K product (256 x 192 x 8192, U[-1,1)) repeats bit for bit and
-    keeps the <= 2x error bound against the oracle, in every mode that
-    claims it."""
-    M, N, K = 256, 192, 8192
+@pytest.mark.parametrize("M,N,K", [(256, 192, 8192), (1024, 1024, 4096)])
+def test_split_k_deterministic_and_within_error_bound(dev, ompblas, M, N, K):
+    """Split-K products (U[-1,1)) repeat bit for bit and keep the <= 2x
+    error bound against the oracle, in every mode that claims it: one with
+    single-block splits (256 x 192 x 8192) and one whose partials would not
+    fit as single blocks, which takes coarser splits (1024 x 1024 x 4096)."""
     a = helpers.rng_uniform(M * K, 7, 0).reshape(M, K)
     b = helpers.rng_uniform(K * N, 7, 1).reshape(K, N)
     truth = a.astype(np.float64) @ b.astype(np.float64)
