@@ -72,6 +72,19 @@ class Timer:
         return e0.elapsed_time(e1) / iters / 1e3  # seconds per call
 
 
+_HOST_CORES = None
+
+
+def host_cores() -> int:
+    """CPUs in this process's affinity mask, as first read (the oracle's
+    OpenMP team size)."""
+    global _HOST_CORES
+    if _HOST_CORES is None:
+        _HOST_CORES = (len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+                       else os.cpu_count() or 1)
+    return _HOST_CORES
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,11 +109,10 @@ def main():
               "(the multi-GPU sharding is a design, not a measured path)", file=sys.stderr)
         sys.exit(2)
 
-    # The oracle's OpenMP team: one thread per core of the process, packed
-    # (read by the OpenMP runtime when it starts, so before any import that
-    # could start it).
-    os.environ.setdefault("OMP_PLACES", "cores")
-    os.environ.setdefault("OMP_PROC_BIND", "close")
+    # The host CPUs this process may use, read before any import that starts
+    # an OpenMP runtime: a runtime told to bind (OMP_PROC_BIND) pins the
+    # main thread to one place, after which the affinity mask shows one CPU.
+    host_cores()
 
     legs = {"plain", "errors", "gemv", "dot", "level2", "grouped", "serving", "c1", "cpu"}
     if args.legs != "all":
@@ -597,7 +609,7 @@ def _grouped_cpu_sample(torch, pools, M, N, K, VM, VN, count=256):
     import helpers
 
     log(f"grouped: CPU oracle on the first {count} GEMMs and {count} GEMVs")
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores = host_cores()
     blk = helpers.host_blocking()
     gi = range(min(count, len(M)))
     vi = range(min(count, len(VM)))
@@ -710,7 +722,7 @@ def c1_leg(pkg, lib, torch, dev, sp, timer):
 
     n = 1024
     log("config 1: mtm 1024^3 (seed 0), oracle and GPU")
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores = host_cores()
     a = helpers.rng_uniform(n * n, 0, 0).reshape(n, n)
     b = helpers.rng_uniform(n * n, 0, 1).reshape(n, n)
     truth = a.astype(np.float64) @ b.astype(np.float64)
@@ -753,7 +765,7 @@ def cpu_reference_legs(pkg, torch, dev, stream, refs=None):
     import helpers
 
     sp = stream.cuda_stream
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores = host_cores()
     blk = helpers.host_blocking()
     out = {"cores": cores}
     n = 16384
@@ -814,7 +826,7 @@ def cpu_baseline(budget_s: float):
     import helpers
 
     n = 8192
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores = host_cores()
     log(f"cpu baseline: oracle mtm 8192^3 on {cores} threads")
     a = helpers.rng_uniform(n * n, 1, 0).reshape(n, n)
     b = helpers.rng_uniform(n * n, 1, 1).reshape(n, n)
