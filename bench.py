@@ -215,6 +215,10 @@ def main():
 
     # split 3-product step time
     t_step3 = timer.run(lambda: gemm(pkg.SPLIT_BF16_P3), max(3, args.steps // 2))
+    # scaled split-FP16 step (rows of A and columns of B scaled by powers of
+    # two, 2 FP16 terms, 3 products on the FP16 MFMA)
+    pkg.check(gemm(pkg.SPLIT_FP16))
+    t_step16 = timer.run(lambda: gemm(pkg.SPLIT_FP16), max(3, args.steps // 2))
 
     # --------------------------------------------------------- plain FP32 --
     plain = {}
@@ -420,6 +424,12 @@ def main():
         "config1_1024": c1,
         "wall_s_timed_region": round(wall, 4),
         "split_bf16_p3_step_ms": round(t_step3 * 1e3, 4),
+        "split_fp16_scaled_p3": {
+            "step_ms": round(t_step16 * 1e3, 4), "tflops": round(flop / t_step16 / 1e12, 3),
+            "speedup_vs_plain": (round(plain["row"]["ms"] / (t_step16 * 1e3), 3)
+                                 if plain else None),
+            "arith": "rows of A / columns of B scaled by powers of two, 2 FP16 terms, "
+                     "3 FP16 MFMA products, FP32 accumulation (errors: errors.*.split_fp16_p3)"},
         "plain_f32": plain,
         "speedup_split_vs_plain": (round(plain["row"]["ms"] / (t_step * 1e3), 3)
                                    if plain else None),
@@ -437,7 +447,8 @@ def main():
 
 def error_leg(pkg, lib, torch, dev, stream, A, B, C):
     """Max- and Frobenius-relative error vs an FP64 product (GPU check kernel)
-    of plain FP32 and split-BF16 (6 and 3 products) at 8192^3, for U[-1,1)
+    of plain FP32, split-BF16 (6 and 3 products) and the scaled split-FP16
+    (3 products) at 8192^3, for U[-1,1)
     and sign*2^u (u ~ U[-8,8]) inputs from seed 2, and the componentwise
     error max |C - C64| / (|A| |B|).  The reference algorithm's (oracle's)
     error is measured on a 64-row slab, whose values do not depend on the
@@ -480,7 +491,8 @@ def error_leg(pkg, lib, torch, dev, stream, A, B, C):
         comp_ref = float(np.max(np.abs(c_ref.astype(np.float64) - truth) / D64[rows].cpu().numpy()))
         d = {"oracle_slab": {"max_rel": ref[0], "frob_rel": ref[1], "componentwise": comp_ref}}
         for mode, name in ((pkg.PLAIN_F32, "plain_f32"), (pkg.SPLIT_BF16, "split_bf16_p6"),
-                           (pkg.SPLIT_BF16_P3, "split_bf16_p3")):
+                           (pkg.SPLIT_BF16_P3, "split_bf16_p3"),
+                           (pkg.SPLIT_FP16, "split_fp16_p3")):
             pkg.check(lib.omb_sgemm(n, n, n, 1.0, A.data_ptr(), n, 1, B.data_ptr(), n, 1, 0.0,
                                     C.data_ptr(), n, 1, mode, sp))
             torch.cuda.synchronize()
@@ -494,7 +506,7 @@ def error_leg(pkg, lib, torch, dev, stream, A, B, C):
                        "slab_ratio_vs_oracle": [slab[0] / ref[0], slab[1] / ref[1]],
                        "slab_componentwise_ratio_vs_oracle": comp_slab / comp_ref}
         p = d["plain_f32"]
-        for name in ("split_bf16_p6", "split_bf16_p3"):
+        for name in ("split_bf16_p6", "split_bf16_p3", "split_fp16_p3"):
             d[name]["ratio_vs_plain_f32"] = [d[name]["max_rel"] / p["max_rel"],
                                              d[name]["frob_rel"] / p["frob_rel"]]
         out[dist] = d

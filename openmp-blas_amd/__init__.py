@@ -41,6 +41,7 @@ EBUSY = -5
 PLAIN_F32 = 0
 SPLIT_BF16 = 1
 SPLIT_BF16_P3 = 2
+SPLIT_FP16 = 3       # rows of A and columns of B scaled by powers of two, 2 FP16 terms, 3 MFMA products
 
 REQ_GEMM, REQ_GEMV, REQ_DOT = 0, 1, 2
 

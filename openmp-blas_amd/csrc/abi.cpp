@@ -147,7 +147,8 @@ extern "C" omb_status omb_sgemm(int64_t M, int64_t N, int64_t K, float alpha, co
         set_error("omb_sgemm: negative extent");
         return OMB_EINVAL;
     }
-    if (mode != OMB_MODE_PLAIN_F32 && mode != OMB_MODE_SPLIT_BF16 && mode != OMB_MODE_SPLIT_BF16_P3) {
+    if (mode != OMB_MODE_PLAIN_F32 && mode != OMB_MODE_SPLIT_BF16 &&
+        mode != OMB_MODE_SPLIT_BF16_P3 && mode != OMB_MODE_SPLIT_FP16) {
         set_error("omb_sgemm: unknown mode " + std::to_string(mode));
         return OMB_EINVAL;
     }
@@ -188,6 +189,8 @@ extern "C" omb_status omb_sgemm(int64_t M, int64_t N, int64_t K, float alpha, co
     if (K == 0) return launch_scale(m, n, beta, C, ldc, s);
     if (mode == OMB_MODE_PLAIN_F32)
         return launch_sgemm_f32(m, n, K, alpha, pa, lda, a_row, pb, ldb, b_row, beta, C, ldc, s);
+    if (mode == OMB_MODE_SPLIT_FP16)
+        return launch_sgemm_split16(m, n, K, alpha, pa, ars, acs, pb, brs, bcs, beta, C, ldc, s);
     return launch_sgemm_split(m, n, K, alpha, pa, ars, acs, pb, brs, bcs, beta, C, ldc,
                               mode == OMB_MODE_SPLIT_BF16_P3 ? 3 : 6, s);
 }

@@ -99,7 +99,22 @@ omb_status launch_split_pack(int64_t R, int64_t K, const float* X, int64_t rs, i
                              __bf16* out, hipStream_t s);
 omb_status launch_sgemm_packed(int64_t M, int64_t N, int64_t K, float alpha, const __bf16* ap,
                                const __bf16* bp, float beta, float* C, int64_t ldc, int nprod,
-                               hipStream_t s, float* part = nullptr);
+                               hipStream_t s, float* part = nullptr, const int* ea = nullptr,
+                               const int* eb = nullptr);
+
+// Scaled split-FP16 SGEMM: every row of A and column of B is scaled by a
+// power of two that brings its largest magnitude into [2^14, 2^15), split
+// into two FP16 pieces, and the
+// three leading products (a0b1 + a1b0 + a0b0) run on the FP16 MFMA; the
+// sums are scaled back by the row and column exponents.  Workspace: the two
+// packed operands (split16_packed_elems each: 2-plane layout), then
+// split16_extra_bytes(M, N), then any split partials.
+int64_t split16_packed_elems(int64_t R, int64_t K);
+size_t split16_extra_bytes(int64_t M, int64_t N);
+omb_status launch_sgemm_split16(int64_t M, int64_t N, int64_t K, float alpha,
+                                const float* A, int64_t rsA, int64_t csA,
+                                const float* B, int64_t rsB, int64_t csB,
+                                float beta, float* C, int64_t ldc, hipStream_t s);
 
 // Scale C (row-major) by beta (K == 0 case).
 omb_status launch_scale(int64_t M, int64_t N, float beta, float* C, int64_t ldc, hipStream_t s);

@@ -504,6 +504,10 @@ omb_status build_plan(const omb_gemm_desc* gemms, int32_t n_gemm, const omb_gemv
                       int32_t n_gemv, int32_t mode, int force_cls, bool capture,
                       omb_group_plan** out) {
     *out = nullptr;
+    if (mode == OMB_MODE_SPLIT_FP16) {
+        set_error("group plan: the scaled FP16 split runs through omb_sgemm only");
+        return OMB_EUNSUPPORTED;
+    }
     if (mode != OMB_MODE_PLAIN_F32 && mode != OMB_MODE_SPLIT_BF16 &&
         mode != OMB_MODE_SPLIT_BF16_P3) {
         set_error("group plan: unknown mode " + std::to_string(mode));
@@ -715,6 +719,10 @@ extern "C" omb_status omb_autotune_group(const omb_gemm_desc* gemms, int32_t n_g
                                          const char* table_path, omb_stream_t stream) {
     clear_error();
     OMB_TRY(validate(gemms, n_gemm, nullptr, 0));
+    if (mode == OMB_MODE_SPLIT_FP16) {
+        set_error("omb_autotune_group: the scaled FP16 split runs through omb_sgemm only");
+        return OMB_EUNSUPPORTED;
+    }
     if (mode != OMB_MODE_PLAIN_F32 && mode != OMB_MODE_SPLIT_BF16 &&
         mode != OMB_MODE_SPLIT_BF16_P3) {
         set_error("omb_autotune_group: unknown mode");

@@ -127,6 +127,11 @@ double flop_of(const omb_request& r) {
 }
 
 omb_status check_request(const omb_request& r, int i) {
+    if (r.kind == OMB_REQ_GEMM && r.mode == OMB_MODE_SPLIT_FP16) {
+        set_error("omb_serve_run: request " + std::to_string(i) +
+                  ": the scaled FP16 split runs through omb_sgemm only");
+        return OMB_EUNSUPPORTED;
+    }
     bool ok = true;
     switch (r.kind) {
         case OMB_REQ_GEMM:

@@ -39,23 +39,40 @@ using namespace split;
 // four at 8192^2.
 constexpr int SPLIT_SL = 2;  // slices per workgroup
 
-// Store 4 consecutive k (kk4 .. kk4+3 within the 64-k tile) of one row.
-__device__ inline void put4(__bf16* lds, int row, int kk4, const float (&v)[4]) {
-    typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-    bf16x4 p0, p1, p2;
+// Store 4 consecutive k (kk4 .. kk4+3 within the 64-k tile) of one row: the
+// three BF16 pieces, or (F16) the two FP16 pieces of the row-scaled values
+// v * 2^e (a slice then holds two planes).
+template <bool F16>
+__device__ inline void put4(__bf16* lds, int row, int kk4, const float (&v)[4], int e) {
+    typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+    u16x4 p0, p1, p2 = {};
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        __bf16 a, b, c;
-        split3(v[e], a, b, c);
-        p0[e] = a;
-        p1[e] = b;
-        p2[e] = c;
+    for (int i = 0; i < 4; ++i) {
+        if (F16) {
+            _Float16 a, b;
+            split2_f16(ldexpf(v[i], e), a, b);
+            p0[i] = __builtin_bit_cast(unsigned short, a);
+            p1[i] = __builtin_bit_cast(unsigned short, b);
+        } else {
+            __bf16 a, b, c;
+            split3(v[i], a, b, c);
+            p0[i] = __builtin_bit_cast(unsigned short, a);
+            p1[i] = __builtin_bit_cast(unsigned short, b);
+            p2[i] = __builtin_bit_cast(unsigned short, c);
+        }
     }
     const int sl = kk4 >> 4, k16 = kk4 & 15;
-    __bf16* base = lds + ((sl * 3) * TM + row) * TK + k16;
-    *reinterpret_cast<bf16x4*>(base) = p0;
-    *reinterpret_cast<bf16x4*>(base + TM * TK) = p1;
-    *reinterpret_cast<bf16x4*>(base + 2 * TM * TK) = p2;
+    __bf16* base = lds + ((sl * (F16 ? 2 : 3)) * TM + row) * TK + k16;
+    *reinterpret_cast<u16x4*>(base) = p0;
+    *reinterpret_cast<u16x4*>(base + TM * TK) = p1;
+    if (!F16) *reinterpret_cast<u16x4*>(base + 2 * TM * TK) = p2;
+}
+
+// Row r's scale exponent for the FP16 split (0 for the BF16 split).
+template <bool F16>
+__device__ inline int row_exp(const unsigned* amax, int64_t r, int64_t R) {
+    if (!F16 || r >= R) return 0;
+    return line_exp(__uint_as_float(amax[r]));
 }
 
 __device__ inline float ld(const float* X, int64_t R, int64_t K, int64_t rs, int64_t cs,
@@ -63,14 +80,22 @@ __device__ inline float ld(const float* X, int64_t R, int64_t K, int64_t rs, int
     return (r < R && k < K) ? X[r * rs + k * cs] : 0.f;
 }
 
+// F16: rows are scaled by 2^e (e from the row's largest magnitude, `amax`
+// as float bits) and split into two FP16 pieces; the workgroups of the
+// first k group also write each row's e to `exps`.
+template <bool F16>
 __global__ __launch_bounds__(NT) void split_pack_kernel(const float* __restrict__ X, int64_t R,
                                                         int64_t K, int64_t rs, int64_t cs,
                                                         int64_t nkt, int k_major,
-                                                        __bf16* __restrict__ out) {
+                                                        __bf16* __restrict__ out,
+                                                        const unsigned* __restrict__ amax,
+                                                        int* __restrict__ exps) {
     __shared__ __attribute__((aligned(16))) __bf16 lds[SPLIT_SL * 3 * TM * TK];  // 24 KB
     const int tid = threadIdx.x;
     const int64_t rt = blockIdx.x, kt0 = (int64_t)blockIdx.y * SPLIT_SL;
     const int64_t r0 = rt * TM, k0 = kt0 * TK;
+    if (F16 && blockIdx.y == 0 && tid < TM && r0 + tid < R)
+        exps[r0 + tid] = row_exp<true>(amax, r0 + tid, R);
     // 128-bit loads (any alignment) need a unit stride along the loaded
     // index; other strides take the element path.  The values are the same.
     const bool vec = k_major ? cs == 1 : rs == 1;
@@ -88,7 +113,7 @@ __global__ __launch_bounds__(NT) void split_pack_kernel(const float* __restrict_
 #pragma unroll
                 for (int e = 0; e < 4; ++e) v[e] = ld(X, R, K, rs, cs, r, k + e);
             }
-            put4(lds, row, kk4, v);
+            put4<F16>(lds, row, kk4, v, row_exp<F16>(amax, r, R));
         }
     } else {
         // (16 * SPLIT_SL) k-columns x 32 float4 along rows: thread -> (4 columns, 4 rows)
@@ -112,15 +137,16 @@ __global__ __launch_bounds__(NT) void split_pack_kernel(const float* __restrict_
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const float v[4] = {m[0][e], m[1][e], m[2][e], m[3][e]};
-                put4(lds, row4 + e, kq * 4, v);
+                put4<F16>(lds, row4 + e, kq * 4, v, row_exp<F16>(amax, r + e, R));
             }
         }
     }
     __syncthreads();
     const int nsl = (int)(nkt - kt0 < SPLIT_SL ? nkt - kt0 : SPLIT_SL);
+    constexpr int slice = (F16 ? 2 : 3) * TM * TK;  // elements per packed slice
     const uint4* src = reinterpret_cast<const uint4*>(lds);
-    uint4* dst = reinterpret_cast<uint4*>(out + (rt * nkt + kt0) * (int64_t)SLICE_A);
-    const int n16 = nsl * SLICE_A / 8;
+    uint4* dst = reinterpret_cast<uint4*>(out + (rt * nkt + kt0) * (int64_t)slice);
+    const int n16 = nsl * slice / 8;
     for (int i = tid; i < n16; i += NT) dst[i] = src[i];
 }
 
@@ -136,44 +162,49 @@ struct MArgs {
     int64_t tiles_m, tiles_n;
     int64_t ktc;  // slices per split (blockIdx.y), a multiple of kFoldSlices
     float* part;  // split partial sums [split][M][N], or null (one split)
+    const int* ea;  // scaled FP16 operands: row exponents of A (else null)
+    const int* eb;  // and column exponents of B
 };
 
-constexpr int VA = SLICE_A / 8 / NT;  // 16-byte loads per thread per slice (3)
-constexpr int VB = SLICE_B / 8 / NT;
-
-// One slice in flight is 6 uint4 per thread (3 of A, 3 of B), kept in named
-// registers (arrays of them were demoted to scratch by the compiler).
+// One slice in flight is PL uint4 of A and PL of B per thread (PL = planes
+// per slice, 3 or 2: one 16-byte load per plane), kept in named registers
+// (arrays of them were demoted to scratch by the compiler).
 #define OMB_GLOAD(kt, r0, r1, r2, r3, r4, r5)                            \
     do {                                                                  \
-        const uint4* a_ = ga + (kt) * (SLICE_A / 8) + tid;                \
-        const uint4* b_ = gb + (kt) * (SLICE_B / 8) + tid;                \
+        const uint4* a_ = ga + (kt) * (SA / 8) + tid;                     \
+        const uint4* b_ = gb + (kt) * (SB / 8) + tid;                     \
         r0 = a_[0];                                                       \
         r1 = a_[NT];                                                      \
-        r2 = a_[2 * NT];                                                  \
+        if constexpr (PL == 3) r2 = a_[2 * NT];                           \
         r3 = b_[0];                                                       \
         r4 = b_[NT];                                                      \
-        r5 = b_[2 * NT];                                                  \
+        if constexpr (PL == 3) r5 = b_[2 * NT];                           \
     } while (0)
 #define OMB_LSTORE(l, r0, r1, r2, r3, r4, r5)                             \
     do {                                                                  \
         (l)[q0] = r0;                                                     \
         (l)[q1] = r1;                                                     \
-        (l)[q2] = r2;                                                     \
+        if constexpr (PL == 3) (l)[q2] = r2;                              \
         (l)[q3] = r3;                                                     \
         (l)[q4] = r4;                                                     \
-        (l)[q5] = r5;                                                     \
+        if constexpr (PL == 3) (l)[q5] = r5;                              \
     } while (0)
 
-// Slice chunk q of A (0 .. 3*TM*2-1): plane q / (2*TM), row (q / 2) % TM,
+// Slice chunk q of A (0 .. PL*TM*2-1): plane q / (2*TM), row (q / 2) % TM,
 // half q % 2 (same for B after the A part) -> its swizzled LDS position.
 __device__ __forceinline__ int lds_index(int q) {
     const int pl = q / (2 * TM), rem = q - pl * 2 * TM;
     return pl * 2 * TM + lds_chunk(rem >> 1, rem & 1);
 }
 
-template <int NPROD>
+// V: bf16x8 for the BF16 pieces (PL = 3 planes per packed slice), f16x8
+// for the scaled FP16 pieces (PL = 2; the sums are scaled back by the row
+// and column exponents when stored).
+template <int NPROD, typename V, int PL>
 __global__ __launch_bounds__(NT, 2) void sgemm_bf16x3_kernel(MArgs p) {
-    __shared__ __attribute__((aligned(16))) uint4 lds[2][(SLICE_A + SLICE_B) / 8];
+    constexpr int SA = PL * TM * TK, SB = PL * TN * TK;  // elements per packed slice
+    static_assert(SA / 8 / NT == PL && SB / 8 / NT == PL, "one 16-byte load per plane");
+    __shared__ __attribute__((aligned(16))) uint4 lds[2][(SA + SB) / 8];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -187,12 +218,11 @@ __global__ __launch_bounds__(NT, 2) void sgemm_bf16x3_kernel(MArgs p) {
     // tile's row and column panels; ktc is a multiple of kFoldSlices, so the
     // accumulation blocks are the same as without the split.
     const int64_t kt0 = (int64_t)blockIdx.y * p.ktc;
-    const uint4* ga = reinterpret_cast<const uint4*>(p.Ap + (tm * p.nkt + kt0) * SLICE_A);
-    const uint4* gb = reinterpret_cast<const uint4*>(p.Bp + (tn * p.nkt + kt0) * SLICE_B);
-    static_assert(VA == 3 && VB == 3, "staging macros assume 3 + 3 loads per slice");
+    const uint4* ga = reinterpret_cast<const uint4*>(p.Ap + (tm * p.nkt + kt0) * SA);
+    const uint4* gb = reinterpret_cast<const uint4*>(p.Bp + (tn * p.nkt + kt0) * SB);
     // LDS slots of this thread's chunks (loop invariant).
     const int q0 = lds_index(tid), q1 = lds_index(tid + NT), q2 = lds_index(tid + 2 * NT);
-    const int q3 = SLICE_A / 8 + q0, q4 = SLICE_A / 8 + q1, q5 = SLICE_A / 8 + q2;
+    const int q3 = SA / 8 + q0, q4 = SA / 8 + q1, q5 = SA / 8 + q2;
 
     f32x16 inner[2][2], outer[2][2];
 #pragma unroll
@@ -201,7 +231,7 @@ __global__ __launch_bounds__(NT, 2) void sgemm_bf16x3_kernel(MArgs p) {
         for (int j = 0; j < 2; ++j) inner[i][j] = outer[i][j] = f32x16{};
     const int frow = lane & 31, fh = lane >> 5;
     auto compute = [&](int buf, int64_t kt) {
-        mfma_slice<NPROD>(lds[buf], wm * 64, wn * 64, frow, fh, inner);
+        mfma_slice<NPROD, 2, 2, TM, TN, V, PL>(lds[buf], wm * 64, wn * 64, frow, fh, inner);
     };
     const int64_t nkt = p.nkt - kt0 < p.ktc ? p.nkt - kt0 : p.ktc;
     auto fold_at = [&](int64_t kt) {
@@ -229,12 +259,97 @@ __global__ __launch_bounds__(NT, 2) void sgemm_bf16x3_kernel(MArgs p) {
         __syncthreads();
     }
 
-    if (p.part)  // raw partial sums of this split (alpha 1, beta 0 store them unchanged)
+    // (The exponents scale a split's partial sums before they are stored:
+    // a power-of-two scale commutes with the in-order sum of the partials.)
+    if (p.part)  // partial sums of this split (alpha 1, beta 0 store them unchanged)
         store_tile(outer, tm * TM + wm * 64, tn * TN + wn * 64, lane,
-                   p.part + (int64_t)blockIdx.y * p.M * p.N, p.N, p.M, p.N, 1.f, 0.f);
+                   p.part + (int64_t)blockIdx.y * p.M * p.N, p.N, p.M, p.N, 1.f, 0.f, p.ea,
+                   p.eb);
     else
         store_tile(outer, tm * TM + wm * 64, tn * TN + wn * 64, lane, p.C, p.ldc, p.M, p.N,
-                   p.alpha, p.beta);
+                   p.alpha, p.beta, p.ea, p.eb);
+}
+
+// ------------------------------------------------- scaled FP16 split --
+// Largest magnitude of every row of an R x K operand (element (r, k) at
+// X[r*rs + k*cs]), as the bits of |x| (for non-negative floats the unsigned
+// order is the numeric order, and NaN's bits exceed infinity's, so a NaN
+// row max is NaN).  The order of a max does not matter: deterministic.
+// k-contiguous rows (cs == 1): one wavefront per row, written directly.
+__global__ __launch_bounds__(NT) void amax_rows_kc_kernel(const float* __restrict__ X,
+                                                          int64_t R, int64_t K, int64_t rs,
+                                                          unsigned* __restrict__ amax) {
+    const int lane = threadIdx.x & 63;
+    const int64_t r = (int64_t)blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+    if (r >= R) return;
+    const float* row = X + r * rs;
+    unsigned m = 0;
+    const int64_t n4 = K >> 2;
+    for (int64_t j = lane; j < n4; j += 64) {
+        const float4 v = ld4u_nt(row + 4 * j);
+        m = max(m, max(max(__float_as_uint(fabsf(v.x)), __float_as_uint(fabsf(v.y))),
+                       max(__float_as_uint(fabsf(v.z)), __float_as_uint(fabsf(v.w)))));
+    }
+    const int64_t t = (n4 << 2) + lane;
+    if (t < K) m = max(m, __float_as_uint(fabsf(row[t])));
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off, 64));
+    if (lane == 0) amax[r] = m;
+}
+
+// Other layouts, over 256-k chunks combined with atomicMax into a zeroed
+// `amax`.  rs == 1 (consecutive rows adjacent in memory): one thread per 4
+// rows, one 128-bit load per k (1 KB per wavefront), 8 k in flight.
+constexpr int kAmaxChunk = 256;
+__global__ __launch_bounds__(NT) void amax_rows4_kernel(const float* __restrict__ X, int64_t R,
+                                                        int64_t K, int64_t cs,
+                                                        unsigned* __restrict__ amax) {
+    const int64_t r = ((int64_t)blockIdx.x * NT + threadIdx.x) * 4;
+    if (r >= R) return;
+    const int64_t k0 = (int64_t)blockIdx.y * kAmaxChunk;
+    const int64_t k1 = k0 + kAmaxChunk < K ? k0 + kAmaxChunk : K;
+    unsigned m[4] = {0u, 0u, 0u, 0u};
+    int64_t k = k0;
+    if (r + 3 < R) {
+        for (; k + 8 <= k1; k += 8) {
+            float4 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = ld4u_nt(X + r + (k + u) * cs);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                m[0] = max(m[0], __float_as_uint(fabsf(v[u].x)));
+                m[1] = max(m[1], __float_as_uint(fabsf(v[u].y)));
+                m[2] = max(m[2], __float_as_uint(fabsf(v[u].z)));
+                m[3] = max(m[3], __float_as_uint(fabsf(v[u].w)));
+            }
+        }
+    }
+    for (; k < k1; ++k)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            if (r + e < R) m[e] = max(m[e], __float_as_uint(fabsf(X[r + e + k * cs])));
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+        if (r + e < R) atomicMax(&amax[r + e], m[e]);
+}
+
+// Any strides: one thread per row.
+__global__ __launch_bounds__(NT) void amax_rows_any_kernel(const float* __restrict__ X,
+                                                           int64_t R, int64_t K, int64_t rs,
+                                                           int64_t cs,
+                                                           unsigned* __restrict__ amax) {
+    const int64_t r = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (r >= R) return;
+    const int64_t k0 = (int64_t)blockIdx.y * kAmaxChunk;
+    const int64_t k1 = k0 + kAmaxChunk < K ? k0 + kAmaxChunk : K;
+    unsigned m = 0;
+    for (int64_t k = k0; k < k1; ++k) m = max(m, __float_as_uint(fabsf(X[r * rs + k * cs])));
+    atomicMax(&amax[r], m);
+}
+
+__global__ __launch_bounds__(NT) void zero_u32_kernel(unsigned* p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
+        p[i] = 0u;
 }
 
 }  // namespace
@@ -242,6 +357,12 @@ __global__ __launch_bounds__(NT, 2) void sgemm_bf16x3_kernel(MArgs p) {
 int64_t split_packed_elems(int64_t R, int64_t K) {
     if (R <= 0 || K <= 0) return 0;
     return 3 * round_up(R, TM) * round_up(K, TK);
+}
+
+// The scaled FP16 form: two planes.
+int64_t split16_packed_elems(int64_t R, int64_t K) {
+    if (R <= 0 || K <= 0) return 0;
+    return 2 * round_up(R, TM) * round_up(K, TK);
 }
 
 omb_status launch_split_pack(int64_t R, int64_t K, const float* X, int64_t rs, int64_t cs,
@@ -254,8 +375,41 @@ omb_status launch_split_pack(int64_t R, int64_t K, const float* X, int64_t rs, i
         return OMB_EUNSUPPORTED;
     }
     const int k_major = cs == 1 ? 1 : 0;
-    hipLaunchKernelGGL(split_pack_kernel, dim3((unsigned)rtiles, (unsigned)kgroups), dim3(NT), 0,
-                       s, X, R, K, rs, cs, nkt, k_major, out);
+    hipLaunchKernelGGL(split_pack_kernel<false>, dim3((unsigned)rtiles, (unsigned)kgroups),
+                       dim3(NT), 0, s, X, R, K, rs, cs, nkt, k_major, out, nullptr, nullptr);
+    OMB_HIP_TRY(hipGetLastError());
+    return OMB_OK;
+}
+
+omb_status launch_split16_pack(int64_t R, int64_t K, const float* X, int64_t rs, int64_t cs,
+                               __bf16* out, unsigned* amax, int* exps, hipStream_t s) {
+    if (R <= 0 || K <= 0) return OMB_OK;
+    const int64_t rtiles = ceil_div(R, TM), nkt = ceil_div(K, TK);
+    const int64_t kgroups = ceil_div(nkt, SPLIT_SL);
+    const int64_t achunks = ceil_div(K, kAmaxChunk);
+    if (rtiles > 0x7fffffffLL || kgroups > 65535 || achunks > 65535) {
+        set_error("split pack: operand too large for one launch");
+        return OMB_EUNSUPPORTED;
+    }
+    if (cs == 1) {
+        hipLaunchKernelGGL(amax_rows_kc_kernel, dim3((unsigned)ceil_div(R, NT / 64)), dim3(NT), 0,
+                           s, X, R, K, rs, amax);
+    } else {
+        hipLaunchKernelGGL(zero_u32_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(R, NT), 1024)),
+                           dim3(NT), 0, s, amax, R);
+        if (rs == 1)
+            hipLaunchKernelGGL(amax_rows4_kernel,
+                               dim3((unsigned)ceil_div(R, 4 * NT), (unsigned)achunks), dim3(NT), 0,
+                               s, X, R, K, cs, amax);
+        else
+            hipLaunchKernelGGL(amax_rows_any_kernel,
+                               dim3((unsigned)ceil_div(R, NT), (unsigned)achunks), dim3(NT), 0, s,
+                               X, R, K, rs, cs, amax);
+    }
+    OMB_HIP_TRY(hipGetLastError());
+    const int k_major = cs == 1 ? 1 : 0;
+    hipLaunchKernelGGL(split_pack_kernel<true>, dim3((unsigned)rtiles, (unsigned)kgroups),
+                       dim3(NT), 0, s, X, R, K, rs, cs, nkt, k_major, out, amax, exps);
     OMB_HIP_TRY(hipGetLastError());
     return OMB_OK;
 }
@@ -272,14 +426,14 @@ int64_t sgemm_split_splits(int64_t M, int64_t N, int64_t K, int64_t* kchunk) {
 
 omb_status launch_sgemm_packed(int64_t M, int64_t N, int64_t K, float alpha, const __bf16* ap,
                                const __bf16* bp, float beta, float* C, int64_t ldc, int nprod,
-                               hipStream_t s, float* part) {
+                               hipStream_t s, float* part, const int* ea, const int* eb) {
     const int64_t tiles_m = ceil_div(M, TM), tiles_n = ceil_div(N, TN);
     if (tiles_m * tiles_n > 0x7fffffffLL) {
         set_error("sgemm split: too many tiles for one launch");
         return OMB_EUNSUPPORTED;
     }
     const int64_t nkt = ceil_div(K, TK);
-    MArgs a{ap, bp, nkt, C, ldc, M, N, alpha, beta, tiles_m, tiles_n, nkt, nullptr};
+    MArgs a{ap, bp, nkt, C, ldc, M, N, alpha, beta, tiles_m, tiles_n, nkt, nullptr, ea, eb};
     // Split K over blockIdx.y when the tiles alone leave CUs idle; the
     // splits' partials are then added in split order.
     int64_t kchunk = K;
@@ -294,10 +448,12 @@ omb_status launch_sgemm_packed(int64_t M, int64_t N, int64_t K, float alpha, con
         a.part = part;
     }
     dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)splits), block(NT);
-    if (nprod == 3)
-        hipLaunchKernelGGL(sgemm_bf16x3_kernel<3>, grid, block, 0, s, a);
+    if (ea)
+        hipLaunchKernelGGL((sgemm_bf16x3_kernel<3, f16x8, 2>), grid, block, 0, s, a);
+    else if (nprod == 3)
+        hipLaunchKernelGGL((sgemm_bf16x3_kernel<3, bf16x8, 3>), grid, block, 0, s, a);
     else
-        hipLaunchKernelGGL(sgemm_bf16x3_kernel<6>, grid, block, 0, s, a);
+        hipLaunchKernelGGL((sgemm_bf16x3_kernel<6, bf16x8, 3>), grid, block, 0, s, a);
     OMB_HIP_TRY(hipGetLastError());
     if (splits > 1) return launch_splitk_reduce(M, N, splits, alpha, beta, part, C, ldc, s);
     return OMB_OK;
@@ -323,6 +479,39 @@ omb_status launch_sgemm_split(int64_t M, int64_t N, int64_t K, float alpha,
     OMB_TRY(launch_split_pack(M, K, A, rsA, csA, ap, s));
     OMB_TRY(launch_split_pack(N, K, B, csB, rsB, bp, s));
     return launch_sgemm_packed(M, N, K, alpha, ap, bp, beta, C, ldc, nprod, s, part);
+}
+
+// Scaled FP16 split: the exponents (M + N ints) and row maxima (M + N
+// unsigned) follow the packed operands in the workspace.
+size_t split16_extra_bytes(int64_t M, int64_t N) {
+    return (size_t)round_up((int64_t)((M + N) * (sizeof(int) + sizeof(unsigned))), 256);
+}
+
+omb_status launch_sgemm_split16(int64_t M, int64_t N, int64_t K, float alpha,
+                                const float* A, int64_t rsA, int64_t csA,
+                                const float* B, int64_t rsB, int64_t csB,
+                                float beta, float* C, int64_t ldc, hipStream_t s) {
+    const size_t a_elems = (size_t)split16_packed_elems(M, K);
+    const size_t b_elems = (size_t)split16_packed_elems(N, K);
+    const size_t packed = round_up((int64_t)((a_elems + b_elems) * sizeof(__bf16)), 256);
+    const size_t extra = split16_extra_bytes(M, N);
+    int64_t kchunk = K;
+    const int64_t splits = sgemm_split_splits(M, N, K, &kchunk);
+    const size_t partial = splits > 1 ? (size_t)(splits * M * N) * sizeof(float) : 0;
+    Workspace* ws = nullptr;
+    OMB_TRY(get_workspace(s, packed + extra + partial, &ws));
+    char* base = static_cast<char*>(ws->buf);
+    __bf16* ap = reinterpret_cast<__bf16*>(base);
+    __bf16* bp = ap + a_elems;
+    int* ea = reinterpret_cast<int*>(base + packed);
+    int* eb = ea + M;
+    unsigned* amax_a = reinterpret_cast<unsigned*>(eb + N);
+    unsigned* amax_b = amax_a + M;
+    float* part = partial ? reinterpret_cast<float*>(base + packed + extra) : nullptr;
+    // A scaled by rows, B^T (rows n) by rows, i.e. B by columns.
+    OMB_TRY(launch_split16_pack(M, K, A, rsA, csA, ap, amax_a, ea, s));
+    OMB_TRY(launch_split16_pack(N, K, B, csB, rsB, bp, amax_b, eb, s));
+    return launch_sgemm_packed(M, N, K, alpha, ap, bp, beta, C, ldc, 3, s, part, ea, eb);
 }
 
 }  // namespace omb

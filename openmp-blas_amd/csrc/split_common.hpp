@@ -8,6 +8,7 @@ namespace omb {
 namespace split {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int TM = 128, TN = 128, TK = 16;  // C tile and k slice of one workgroup
@@ -19,6 +20,9 @@ constexpr int kFoldSlices = 16;              // inner MFMA accumulation spans 25
 
 __device__ inline f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+__device__ inline f32x16 mfma(f16x8 a, f16x8 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
 }
 
 // LDS position (in 16-byte chunks, within one plane) of chunk h (0/1) of
@@ -40,6 +44,21 @@ __device__ inline void split3(float x, __bf16& h0, __bf16& h1, __bf16& h2) {
     const float r1 = x - (float)h0;
     h1 = (__bf16)r1;
     h2 = (__bf16)(r1 - (float)h1);
+}
+
+// Scaled FP16 split (two planes per packed slice): the power-of-two exponent that brings a line whose
+// largest magnitude is amax into [2^14, 2^15) (0 for a zero, infinite or
+// NaN amax), and x * 2^e = h0 + h1 + r with h0 = fp16(x 2^e) and
+// h1 = fp16(x 2^e - h0) (the subtraction is exact), |r| <= 2^-22 |x 2^e| for
+// elements at most 2^10 below amax, and |r| <= 2^-25 below that (FP16's
+// subnormal step).
+__device__ inline int line_exp(float amax) {
+    if (!(amax > 0.f) || __builtin_isinf(amax)) return 0;
+    return 14 - ilogbf(amax);
+}
+__device__ inline void split2_f16(float xs, _Float16& h0, _Float16& h1) {
+    h0 = (_Float16)xs;
+    h1 = (_Float16)(xs - (float)h0);
 }
 
 // Split 8 consecutive-k values into the three 16-byte plane chunks.
@@ -69,24 +88,29 @@ namespace split {
 // (rows wrow0.., columns wcol0.. of the slice): fragments from the swizzled
 // LDS slice (PA rows per A plane, PB per B plane), smallest products first,
 // all into the inner accumulators.
-template <int NPROD, int BI = 2, int BJ = 2, int PA = TM, int PB = TN>
+// V is the 8-element operand vector of the MFMA: bf16x8 (BF16 pieces) or
+// f16x8 (scaled FP16 pieces, 3 products).
+// PLA: planes of the A part of the LDS slice (3, or 2 for the FP16 form).
+template <int NPROD, int BI = 2, int BJ = 2, int PA = TM, int PB = TN, typename V = bf16x8,
+          int PLA = 3>
 __device__ __forceinline__ void mfma_slice(const uint4* slice, int wrow0, int wcol0, int frow,
                                            int fh, f32x16 (&inner)[BI][BJ]) {
     constexpr int NPLANES = NPROD == 6 ? 3 : 2;
+    static_assert(NPLANES <= PLA, "the slice holds every plane the products use");
     const uint4* la = slice;
-    const uint4* lb = slice + 3 * 2 * PA;
-    bf16x8 a[NPLANES][BI], b[NPLANES][BJ];
+    const uint4* lb = slice + PLA * 2 * PA;
+    V a[NPLANES][BI], b[NPLANES][BJ];
 #pragma unroll
     for (int pl = 0; pl < NPLANES; ++pl) {
 #pragma unroll
         for (int i = 0; i < BI; ++i) {
             uint4 va = la[pl * 2 * PA + lds_chunk(wrow0 + i * 32 + frow, fh)];
-            a[pl][i] = *reinterpret_cast<const bf16x8*>(&va);
+            a[pl][i] = *reinterpret_cast<const V*>(&va);
         }
 #pragma unroll
         for (int j = 0; j < BJ; ++j) {
             uint4 vb = lb[pl * 2 * PB + lds_chunk(wcol0 + j * 32 + frow, fh)];
-            b[pl][j] = *reinterpret_cast<const bf16x8*>(&vb);
+            b[pl][j] = *reinterpret_cast<const V*>(&vb);
         }
     }
 #pragma unroll
@@ -119,23 +143,29 @@ __device__ __forceinline__ void fold(f32x16 (&inner)[BI][BJ], f32x16 (&outer)[BI
 
 // C = alpha * acc + beta * C for the wavefront's BI x BJ blocks (row-major C).
 // 32x32 accumulator layout: lane l holds column l % 32, value r holds row
-// 8*(r/4) + 4*(l/32) + r%4.
+// 8*(r/4) + 4*(l/32) + r%4.  With row and column exponents (scaled FP16
+// operands: row m of A was scaled by 2^ea[m], column n of B by 2^eb[n]),
+// acc is first scaled back by 2^-(ea[m] + eb[n]), exactly unless the result
+// leaves FP32's normal range (as the FP32 product itself would).
 template <int BI, int BJ>
 __device__ __forceinline__ void store_tile(const f32x16 (&acc)[BI][BJ], int64_t m_base,
                                            int64_t n_base, int lane, float* C, int64_t ldc,
-                                           int64_t M, int64_t N, float alpha, float beta) {
+                                           int64_t M, int64_t N, float alpha, float beta,
+                                           const int* ea = nullptr, const int* eb = nullptr) {
 #pragma unroll
     for (int i = 0; i < BI; ++i)
 #pragma unroll
         for (int j = 0; j < BJ; ++j) {
             const int64_t n = n_base + j * 32 + (lane & 31);
             if (n >= N) continue;
+            const int en = ea ? eb[n] : 0;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int64_t m = m_base + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
                 if (m < M) {
                     float* c = C + m * ldc + n;
-                    const float v = alpha * acc[i][j][r];
+                    const float a = ea ? ldexpf(acc[i][j][r], -(ea[m] + en)) : acc[i][j][r];
+                    const float v = alpha * a;
                     *c = beta != 0.f ? v + beta * *c : v;
                 }
             }

@@ -115,6 +115,10 @@ def test_group_plan_rejects_bad_descriptors(ompblas):
     badv = (ompblas.GemvDesc * 1)(ompblas.GemvDesc(4, 8, 16, 4, 16, 16, 1.0, 0.0))       # lda < N
     assert lib.omb_group_plan_create(None, 0, badv, 1, 0, None, ctypes.byref(h)) == -1
     assert lib.omb_group_plan_create(None, 0, None, 0, 9, None, ctypes.byref(h)) == -1   # mode
+    # the scaled FP16 split is a direct-call mode
+    assert lib.omb_group_plan_create(None, 0, None, 0, ompblas.SPLIT_FP16, None,
+                                     ctypes.byref(h)) == -2
+    assert lib.omb_autotune_group(None, 0, ompblas.SPLIT_FP16, None, None) == -2
     assert lib.omb_group_plan_create(None, -1, None, 0, 0, None, ctypes.byref(h)) == -1
     assert "group plan" in ompblas.last_error()
     assert lib.omb_autotune_load(b"/nonexistent/table.txt") == -1
@@ -140,6 +144,7 @@ def test_serve_run_rejects_bad_arguments(ompblas):
     assert run([ok], cfg=Cfg(2, -5, 0, 1)) == -1                     # negative budget
     assert run([R(7, 0, 1, 1, 1, 0.0, 16, 16, 16, 0)]) == -1         # unknown kind
     assert run([R(ompblas.REQ_GEMM, 9, 4, 4, 4, 0.0, 16, 16, 16, 0)]) == -1   # bad mode
+    assert run([R(ompblas.REQ_GEMM, ompblas.SPLIT_FP16, 4, 4, 4, 0.0, 16, 16, 16, 0)]) == -2
     assert run([R(ompblas.REQ_GEMM, 1, 4, 4, 4, 0.0, 16, None, 16, 0)]) == -1  # missing B
     assert run([R(ompblas.REQ_GEMV, 0, 4, 4, 0, 0.0, 16, 16, None, 0)]) == -1  # missing y
     assert run([ok, R(ompblas.REQ_DOT, 0, 0, 8, 0, -1.0, 16, 16, 16, 0)]) == -1  # arrival < 0

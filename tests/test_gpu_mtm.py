@@ -15,7 +15,21 @@ from helpers import oracle_mtm, rand100, rel_errors
 pytestmark = pytest.mark.gpu
 
 LAYOUTS = ["FFF", "FFL", "FLF", "LFF", "FLL", "LFL", "LLF", "LLL"]
-MODES = [0, 1, 2]  # plain FP32, split-BF16 (6 products), split-BF16 (3 products)
+# plain FP32, split-BF16 (6 products), split-BF16 (3 products), scaled split-FP16 (3 products)
+MODES = [0, 1, 2, 3]
+ACCURATE = (0, 1, 3)  # the modes that keep the <= 2x error bound
+
+
+def split16_bound(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """Error bound of the scaled split-FP16 product, per element: FP32-level
+    relative accuracy on |A||B| (2^-20 leaves room for the accumulation at
+    these sizes) plus, for an element far below the largest of its row of A
+    (column of B), FP16's subnormal step: an absolute 2^-39 of that largest,
+    twice over for margin."""
+    aa, bb = np.abs(a).astype(np.float64), np.abs(b).astype(np.float64)
+    return (2.0 ** -20 * (aa @ bb)
+            + 2.0 ** -38 * aa.max(axis=1, keepdims=True) * bb.sum(axis=0, keepdims=True)
+            + 2.0 ** -38 * aa.sum(axis=1, keepdims=True) * bb.max(axis=0, keepdims=True))
 
 
 def to_dev(a: np.ndarray, dev) -> torch.Tensor:
@@ -40,7 +54,8 @@ def run_mtm(ompblas, c, a, b, mode, alpha=1.0, beta=1.0):
 @pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("combo", LAYOUTS)
 def test_mtm_integer_exact_reference_cases(dev, ompblas, combo, mode):
-    """Values 0..99 are exact in one BF16 term, so every mode is exact here."""
+    """Values 0..99 are exact in one BF16 term and, scaled by a power of two,
+    in one FP16 term, so every mode is exact here."""
     order = {"F": "F", "L": "C"}
     oc, oa, ob = (order[ch] for ch in combo)
     for sz in range(2, 32):
@@ -157,7 +172,7 @@ def test_split_k_deterministic_and_within_error_bound(dev, ompblas, M, N, K):
     oracle_mtm(c_ref, a, b)
     ref = rel_errors(c_ref, truth)
     ta, tb = to_dev(a, dev), to_dev(b, dev)
-    for mode in (0, 1):
+    for mode in ACCURATE:
         outs = []
         for _ in range(2):
             tc = torch.zeros(M, N, device=dev)
@@ -217,8 +232,47 @@ def test_split_values_beyond_bf16_range_stay_finite(dev, ompblas, orders):
         run_mtm(ompblas, tc, ta, tb, mode)
         got = tc.cpu().numpy().astype(np.float64)
         assert np.isfinite(got).all(), mode
+        if mode == 3:
+            # Rows scaled by their largest element: the small entries beside a
+            # FLT_MAX-sized one keep only an absolute accuracy of 2^-39 of it.
+            bound = split16_bound(a, b)
+            assert np.all(np.abs(got - truth) <= bound), mode
+            continue
         tol = 1e-6 if mode < 2 else 2e-5
         assert np.all(np.abs(got - truth) <= tol * np.abs(truth) + 1e-3), mode
+
+
+@pytest.mark.parametrize("orders", ["CC", "FF", "CF"])
+def test_split_fp16_rows_and_columns_of_any_scale(dev, ompblas, orders):
+    """The scaled FP16 split normalises every row of A and column of B by a
+    power of two, so rows and columns 2^+-60 apart (products up to 2^120) are
+    as accurate as unscaled ones: componentwise within the split16 bound and
+    within 2x plain FP32's error against FP64 per element class.  All-zero
+    rows and columns (scale exponent 0) give exact zeros."""
+    M, K, N = 96, 700, 80
+    rng = np.random.default_rng(5)
+    a = rng.uniform(-1, 1, (M, K)).astype(np.float32)
+    b = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+    a *= (2.0 ** rng.choice([-60, 0, 60], size=(M, 1))).astype(np.float32)
+    b *= (2.0 ** rng.choice([-60, 0, 60], size=(1, N))).astype(np.float32)
+    a[7, :] = 0.0
+    b[:, 11] = 0.0
+    a = np.asarray(a, order=orders[0])
+    b = np.asarray(b, order=orders[1])
+    truth = a.astype(np.float64) @ b.astype(np.float64)
+    ta, tb = to_dev(a, dev), to_dev(b, dev)
+    got = {}
+    for mode in (0, 3):
+        tc = torch.zeros(M, N, device=dev)
+        run_mtm(ompblas, tc, ta, tb, mode)
+        got[mode] = tc.cpu().numpy().astype(np.float64)
+    assert np.all(np.abs(got[3] - truth) <= split16_bound(a, b))
+    assert (got[3][7, :] == 0).all() and (got[3][:, 11] == 0).all()
+    # per element, relative to |A||B|: the scaled split is no worse than 2x plain
+    absab = np.abs(a).astype(np.float64) @ np.abs(b).astype(np.float64)
+    nz = absab > 0
+    err = {m: (np.abs(got[m] - truth)[nz] / absab[nz]).max() for m in (0, 3)}
+    assert err[3] <= 2 * err[0], err
 
 
 def _fp_case(dev, ompblas, M, N, K, seed, c_order="C", a_order="C", b_order="C"):
@@ -243,7 +297,7 @@ def test_mtm_config1_error_vs_reference(dev, ompblas):
     reference algorithm's (oracle, this host's KC)."""
     ref, out = _fp_case(dev, ompblas, 1024, 1024, 1024, 0)
     print("oracle", ref, "gpu", out)
-    for mode in (0, 1):
+    for mode in ACCURATE:
         assert out[mode][0] <= 2 * ref[0], (mode, out[mode], ref)
         assert out[mode][1] <= 2 * ref[1], (mode, out[mode], ref)
     assert out[2][1] < 1e-4   # 3-product variant: only a loose sanity bound
@@ -252,7 +306,7 @@ def test_mtm_config1_error_vs_reference(dev, ompblas):
 @pytest.mark.parametrize("orders", [("F", "F", "F"), ("C", "F", "C"), ("F", "C", "F")])
 def test_mtm_mixed_layout_error(dev, ompblas, orders):
     ref, out = _fp_case(dev, ompblas, 777, 555, 1999, 13, *orders)
-    for mode in (0, 1):
+    for mode in ACCURATE:
         assert out[mode][1] <= 2 * ref[1], (mode, out[mode], ref)
 
 
@@ -287,7 +341,7 @@ def test_mtm_8192_slab_error_vs_reference(dev, ompblas, layout):
     ref = rel_errors(c_ref, truth)
     ta, tb = to_dev(a, dev), to_dev(b, dev)
     del a, b
-    for mode in (0, 1):
+    for mode in ACCURATE:
         tc = to_dev(np.zeros((n, n), np.float32, order=order), dev)
         run_mtm(ompblas, tc, ta, tb, mode)
         got = rel_errors(tc[rows].cpu().numpy(), truth)
