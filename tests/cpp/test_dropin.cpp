@@ -475,15 +475,17 @@ int time_host() {
         b[i] = (float)(std::rand() % 100) / 100.f;
     }
     const std::size_t na[2] = {n, n}, wrow[2] = {n, 1};
-    for (int mode : {OMB_MODE_PLAIN_F32, OMB_MODE_SPLIT_BF16}) {
+    for (int mode : {OMB_MODE_PLAIN_F32, OMB_MODE_SPLIT_BF16, OMB_MODE_SPLIT_FP16}) {
         amt::dropin::settings().gemm_mode = mode;
         const double t = host_call_seconds([&] {
             amt::mtm_helper(c.data(), na, wrow, a.data(), na, wrow, b.data(), na, wrow,
                             lay::last_order{});
         });
         std::printf("host mtm 8192^3 %s: %.3f s, %.2f TFLOP/s (768 MiB to the device, 256 MiB back)\n",
-                    mode == OMB_MODE_PLAIN_F32 ? "plain" : "split-bf16", t,
-                    2.0 * n * n * n / t / 1e12);
+                    mode == OMB_MODE_PLAIN_F32   ? "plain"
+                    : mode == OMB_MODE_SPLIT_BF16 ? "split-bf16"
+                                                  : "split-fp16",
+                    t, 2.0 * n * n * n / t / 1e12);
     }
     amt::dropin::settings().gemm_mode = OMB_MODE_PLAIN_F32;
     const std::size_t m = 16384;
@@ -536,6 +538,8 @@ int main(int argc, char** argv) {
         test_errors();
         amt::dropin::settings().gemm_mode = OMB_MODE_SPLIT_BF16;
         for (std::size_t sz : {5, 31}) mtm_case<amt::layout::last_order>(false, true, false, sz, false, "LFL split");
+        amt::dropin::settings().gemm_mode = OMB_MODE_SPLIT_FP16;
+        for (std::size_t sz : {5, 31}) mtm_case<amt::layout::last_order>(false, true, false, sz, false, "LFL split-fp16");
         amt::dropin::settings().gemm_mode = OMB_MODE_PLAIN_F32;
     } catch (const std::exception& e) {
         std::printf("FAIL: exception: %s\n", e.what());
