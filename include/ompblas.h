@@ -41,9 +41,10 @@ typedef void* omb_stream_t; /* hipStream_t */
 #define OMB_MODE_SPLIT_BF16_P3 2 /* same split, 3 leading products only (faster, less accurate) */
 /* Every row of A and column of B scaled by a power of two (largest magnitude
  * into [2^14, 2^15)) and split into 2 FP16 terms; the 3 leading products on
- * the FP16 MFMA, FP32 accumulation, sums scaled back.  Normwise as accurate
- * as plain FP32; an element more than 2^24 below the largest of its row (A)
- * or column (B) keeps only an absolute accuracy of 2^-39 of that largest.
+ * the FP16 MFMA, FP32 accumulation, sums scaled back.  Each element x is
+ * carried to within max(2^-22 |x|, 2^-39 amax), amax the largest magnitude
+ * of its row (A) or column (B): normwise as accurate as plain FP32, while an
+ * element far below amax keeps only that absolute accuracy.
  * omb_sgemm only (not the grouped or serving entry points). */
 #define OMB_MODE_SPLIT_FP16 3
 
