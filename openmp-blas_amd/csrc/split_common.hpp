@@ -46,12 +46,11 @@ __device__ inline void split3(float x, __bf16& h0, __bf16& h1, __bf16& h2) {
     h2 = (__bf16)(r1 - (float)h1);
 }
 
-// Scaled FP16 split (two planes per packed slice): the power-of-two exponent that brings a line whose
-// largest magnitude is amax into [2^14, 2^15) (0 for a zero, infinite or
-// NaN amax), and x * 2^e = h0 + h1 + r with h0 = fp16(x 2^e) and
-// h1 = fp16(x 2^e - h0) (the subtraction is exact), |r| <= 2^-22 |x 2^e| for
-// elements at most 2^10 below amax, and |r| <= 2^-25 below that (FP16's
-// subnormal step).
+// Scaled FP16 split (two planes per packed slice): the power-of-two exponent
+// that brings a line whose largest magnitude is amax into [2^14, 2^15) (0
+// for a zero, infinite or NaN amax), and x * 2^e = h0 + h1 + r with
+// h0 = fp16(x 2^e) and h1 = fp16(x 2^e - h0) (the subtraction is exact),
+// |r| <= max(2^-22 |x 2^e|, 2^-25): relative to x, or FP16's subnormal step.
 __device__ inline int line_exp(float amax) {
     if (!(amax > 0.f) || __builtin_isinf(amax)) return 0;
     return 14 - ilogbf(amax);
