@@ -1,8 +1,9 @@
 // Split-BF16 SGEMM for gfx950: an FP32 GEMM computed on the BF16 MFMA cores.
 //
-// Every FP32 operand element x is split exactly into three BF16 terms
+// Every FP32 operand element x is split into three BF16 terms
 //   x0 = bf16(x), x1 = bf16(x - x0), x2 = bf16(x - x0 - x1)
-// (each subtraction is exact in FP32; 3 x 8 significand bits cover FP32's 24),
+// (each subtraction is exact in FP32; 3 x 8 significand bits cover FP32's 24,
+// so the split is exact for |x| >= 2^-110, and within 2^-134 below),
 // and  a*b ~ a0b0 + (a0b1 + a1b0) + (a0b2 + a1b1 + a2b0)  -- 6 products, the
 // dropped terms are below 2^-24 relative.  The 3-product variant keeps only
 // a0b0 + a0b1 + a1b0.

@@ -33,10 +33,12 @@ __device__ inline int lds_chunk(int r, int h) {
     return (r << 1) | (h ^ ((r >> 2) & 1) ^ ((r >> 3) & 1));
 }
 
-// x = h0 + h1 + h2 exactly (each subtraction is exact in FP32).  A finite x
-// above BF16's largest finite value (|x| > 0x1.fep127 rounds to infinity)
-// takes h0 truncated instead, so the split stays finite and exact.  An
-// infinite or NaN x gives NaN pieces: every output it feeds is NaN.
+// x = h0 + h1 + h2 exactly for |x| >= 2^-110 and zero (each subtraction is
+// exact in FP32); below 2^-110 the last piece rounds to BF16's subnormal
+// step, within 2^-134.  A finite x above BF16's largest finite value
+// (|x| > 0x1.fep127 rounds to infinity) takes h0 truncated instead, so the
+// split stays finite and exact.  An infinite or NaN x gives NaN pieces:
+// every output it feeds is NaN.
 __device__ inline void split3(float x, __bf16& h0, __bf16& h1, __bf16& h2) {
     h0 = (__bf16)x;
     if (__builtin_isinf((float)h0) && !__builtin_isinf(x))
