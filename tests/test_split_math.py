@@ -6,29 +6,45 @@ import numpy as np
 import pytest
 
 
-def bf16_rne(x: np.ndarray) -> np.ndarray:
-    """float32 -> nearest BF16 (ties to even), returned as float32 (finite x)."""
-    b = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
-    r = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16) & 0xFFFF
-    return (r << 16).astype(np.uint32).view(np.float32)
+BF16_MAX = float.fromhex("0x1.fep127")
+
+
+def bf16_round(v: np.ndarray) -> np.ndarray:
+    """Nearest BF16 value (ties to even) of float64 values that are exact
+    FP32 values, in FP64 arithmetic: 8 significant bits, and BF16's
+    subnormal step 2^-133 below 2^-126.  (FP64, not FP32 bit tricks: a
+    process that loaded code built with -ffast-math flushes FP32 subnormals.)
+    Values that round past BF16's largest become infinite."""
+    v = np.asarray(v, np.float64)
+    a = np.abs(v)
+    e = np.floor(np.log2(np.where(a > 0, a, 1.0)))
+    q = 2.0 ** (np.maximum(e, -126) - 7)
+    r = np.where(a > 0, np.round(v / q) * q, v)
+    return np.where(np.abs(r) > BF16_MAX, np.copysign(np.inf, v), r)
 
 
 def split3_bf16(x: np.ndarray):
-    """h0 = bf16(x) (truncated when x is finite but rounds past BF16's
-    largest), h1 = bf16(x - h0), h2 = bf16(x - h0 - h1); FP32 subtractions."""
-    x = np.asarray(x, np.float32)
-    h0 = bf16_rne(x)
-    over = np.isinf(h0) & ~np.isinf(x)
-    h0 = np.where(over, (x.view(np.uint32) & np.uint32(0xFFFF0000)).view(np.float32), h0)
-    r1 = (x - h0).astype(np.float32)
-    h1 = bf16_rne(r1)
-    h2 = bf16_rne((r1 - h1).astype(np.float32))
+    """h0 = bf16(x) (truncated to 8 bits when x is finite but rounds past
+    BF16's largest), h1 = bf16(x - h0), h2 = bf16(x - h0 - h1); the
+    subtractions are exact, so FP64 gives the FP32 values."""
+    x = np.asarray(x)
+    x = x if x.dtype == np.float64 else x.astype(np.float64)   # values on FP32's grid
+    h0 = bf16_round(x)
+    over = np.isinf(h0) & np.isfinite(x)
+    q = 2.0 ** (np.floor(np.log2(np.where(over, np.abs(x), 1.0))) - 7)
+    h0 = np.where(over, np.trunc(x / q) * q, h0)
+    r1 = x - h0
+    h1 = bf16_round(r1)
+    h2 = bf16_round(r1 - h1)
     return h0, h1, h2
 
 
 def signed_pow2_samples(rng, e: int, n: int = 20000) -> np.ndarray:
+    """FP32 values of magnitude in [2^e, 2^(e+1)), as float64 (rounded to
+    FP32's grid in FP64, subnormals included)."""
     m = rng.uniform(1, 2, n) * np.where(rng.random(n) < 0.5, -1.0, 1.0)
-    return (m * 2.0 ** e).astype(np.float32)
+    q = 2.0 ** (max(e, -126) - 23)                       # FP32 spacing there
+    return np.round(m * 2.0 ** e / q) * q
 
 
 @pytest.mark.parametrize("e", [127, 100, 0, -1, -60, -100, -110])
@@ -36,7 +52,7 @@ def test_bf16_split_is_exact_down_to_2_pow_minus_110(e):
     rng = np.random.default_rng(e + 1000)
     x = signed_pow2_samples(rng, e)
     h0, h1, h2 = split3_bf16(x)
-    s = h0.astype(np.float64) + h1.astype(np.float64) + h2.astype(np.float64)
+    s = h0 + h1 + h2
     assert np.array_equal(s, x.astype(np.float64))
     # the pieces fall by at least 2^8 each
     nz = h1 != 0
@@ -50,8 +66,7 @@ def test_bf16_split_below_2_pow_minus_110_is_within_2_pow_minus_134(e):
     rng = np.random.default_rng(-e)
     x = signed_pow2_samples(rng, e)
     h0, h1, h2 = split3_bf16(x)
-    s = h0.astype(np.float64) + h1.astype(np.float64) + h2.astype(np.float64)
-    err = np.abs(s - x.astype(np.float64))
+    err = np.abs(h0 + h1 + h2 - x.astype(np.float64))
     assert err.max() <= 2.0 ** -134
     if e == -111:
         assert err.max() > 0        # the bound is reached: not exact below 2^-110
@@ -62,8 +77,7 @@ def test_bf16_split_of_values_beyond_bf16_range_is_exact():
     x = np.array([fmax, -fmax, 3.3965e38, -3.397e38, 3.39e38], np.float32)
     h0, h1, h2 = split3_bf16(x)
     assert np.isfinite(h0).all()
-    s = h0.astype(np.float64) + h1.astype(np.float64) + h2.astype(np.float64)
-    assert np.array_equal(s, x.astype(np.float64))
+    assert np.array_equal(h0 + h1 + h2, x.astype(np.float64))
 
 
 def scaled_split16(X: np.ndarray, axis: int):
@@ -115,9 +129,8 @@ def test_fp16_three_products_are_below_fp32_rounding(dist):
     rel = np.abs(C3 - T).max() / np.abs(T).max()
     assert rel < 1e-7, rel
     # the BF16 split needs six products for the same: three are 2^-16-level short
-    h = [split3_bf16(A), split3_bf16(B.T)]
-    a = [p.astype(np.float64) for p in h[0]]
-    b = [p.astype(np.float64).T for p in h[1]]
+    a = list(split3_bf16(A))
+    b = [p.T for p in split3_bf16(B.T)]
     C3b = a[0] @ b[1] + a[1] @ b[0] + a[0] @ b[0]
     C6b = C3b + a[0] @ b[2] + a[1] @ b[1] + a[2] @ b[0]
     assert np.abs(C6b - T).max() / np.abs(T).max() < 1e-7
