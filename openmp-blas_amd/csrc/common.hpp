@@ -93,14 +93,16 @@ omb_status launch_sgemm_split(int64_t M, int64_t N, int64_t K, float alpha,
                               const float* B, int64_t rsB, int64_t csB,
                               float beta, float* C, int64_t ldc, int nprod, hipStream_t s);
 
-// Packed split-BF16 operands (3 planes, 128-row x 16-k tiles, zero padded).
+// Packed split-BF16 operands (3 planes, 128-row x 16-k tiles, zero padded;
+// planes = 2 keeps the first two, all the 3-product form reads, in
+// split16_packed_elems elements).
 int64_t split_packed_elems(int64_t R, int64_t K);
 omb_status launch_split_pack(int64_t R, int64_t K, const float* X, int64_t rs, int64_t cs,
-                             __bf16* out, hipStream_t s);
+                             __bf16* out, hipStream_t s, int planes = 3);
 omb_status launch_sgemm_packed(int64_t M, int64_t N, int64_t K, float alpha, const __bf16* ap,
                                const __bf16* bp, float beta, float* C, int64_t ldc, int nprod,
                                hipStream_t s, float* part = nullptr, const int* ea = nullptr,
-                               const int* eb = nullptr);
+                               const int* eb = nullptr, int planes = 3);
 
 // Scaled split-FP16 SGEMM: every row of A and column of B is scaled by a
 // power of two that brings its largest magnitude into [2^14, 2^15), split

@@ -41,9 +41,10 @@ using namespace split;
 constexpr int SPLIT_SL = 2;  // slices per workgroup
 
 // Store 4 consecutive k (kk4 .. kk4+3 within the 64-k tile) of one row: the
-// three BF16 pieces, or (F16) the two FP16 pieces of the row-scaled values
-// v * 2^e (a slice then holds two planes).
-template <bool F16>
+// BF16 pieces (PLN = 3 planes, or 2 for the 3-product form, which never
+// reads the third), or (F16, PLN = 2) the two FP16 pieces of the
+// row-scaled values v * 2^e.
+template <bool F16, int PLN>
 __device__ inline void put4(__bf16* lds, int row, int kk4, const float (&v)[4], int e) {
     typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
     u16x4 p0, p1, p2 = {};
@@ -63,10 +64,10 @@ __device__ inline void put4(__bf16* lds, int row, int kk4, const float (&v)[4], 
         }
     }
     const int sl = kk4 >> 4, k16 = kk4 & 15;
-    __bf16* base = lds + ((sl * (F16 ? 2 : 3)) * TM + row) * TK + k16;
+    __bf16* base = lds + ((sl * PLN) * TM + row) * TK + k16;
     *reinterpret_cast<u16x4*>(base) = p0;
     *reinterpret_cast<u16x4*>(base + TM * TK) = p1;
-    if (!F16) *reinterpret_cast<u16x4*>(base + 2 * TM * TK) = p2;
+    if (PLN == 3) *reinterpret_cast<u16x4*>(base + 2 * TM * TK) = p2;
 }
 
 // Row r's scale exponent for the FP16 split (0 for the BF16 split).
@@ -84,7 +85,7 @@ __device__ inline float ld(const float* X, int64_t R, int64_t K, int64_t rs, int
 // F16: rows are scaled by 2^e (e from the row's largest magnitude, `amax`
 // as float bits) and split into two FP16 pieces; the workgroups of the
 // first k group also write each row's e to `exps`.
-template <bool F16>
+template <bool F16, int PLN>
 __global__ __launch_bounds__(NT) void split_pack_kernel(const float* __restrict__ X, int64_t R,
                                                         int64_t K, int64_t rs, int64_t cs,
                                                         int64_t nkt, int k_major,
@@ -114,7 +115,7 @@ __global__ __launch_bounds__(NT) void split_pack_kernel(const float* __restrict_
 #pragma unroll
                 for (int e = 0; e < 4; ++e) v[e] = ld(X, R, K, rs, cs, r, k + e);
             }
-            put4<F16>(lds, row, kk4, v, row_exp<F16>(amax, r, R));
+            put4<F16, PLN>(lds, row, kk4, v, row_exp<F16>(amax, r, R));
         }
     } else {
         // (16 * SPLIT_SL) k-columns x 32 float4 along rows: thread -> (4 columns, 4 rows)
@@ -138,13 +139,15 @@ __global__ __launch_bounds__(NT) void split_pack_kernel(const float* __restrict_
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const float v[4] = {m[0][e], m[1][e], m[2][e], m[3][e]};
-                put4<F16>(lds, row4 + e, kq * 4, v, row_exp<F16>(amax, r + e, R));
+                put4<F16, PLN>(lds, row4 + e, kq * 4, v, row_exp<F16>(amax, r + e, R));
             }
         }
     }
     __syncthreads();
     const int nsl = (int)(nkt - kt0 < SPLIT_SL ? nkt - kt0 : SPLIT_SL);
-    constexpr int slice = (F16 ? 2 : 3) * TM * TK;  // elements per packed slice
+    static_assert(PLN == 3 || PLN == 2, "3 planes, or the 2 a 3-product form reads");
+    static_assert(!F16 || PLN == 2, "the FP16 split has two pieces");
+    constexpr int slice = PLN * TM * TK;  // elements per packed slice
     const uint4* src = reinterpret_cast<const uint4*>(lds);
     uint4* dst = reinterpret_cast<uint4*>(out + (rt * nkt + kt0) * (int64_t)slice);
     const int n16 = nsl * slice / 8;
@@ -360,14 +363,15 @@ int64_t split_packed_elems(int64_t R, int64_t K) {
     return 3 * round_up(R, TM) * round_up(K, TK);
 }
 
-// The scaled FP16 form: two planes.
+// The two-plane form (the scaled FP16 pieces, or the two BF16 pieces the
+// 3-product form reads).
 int64_t split16_packed_elems(int64_t R, int64_t K) {
     if (R <= 0 || K <= 0) return 0;
     return 2 * round_up(R, TM) * round_up(K, TK);
 }
 
 omb_status launch_split_pack(int64_t R, int64_t K, const float* X, int64_t rs, int64_t cs,
-                             __bf16* out, hipStream_t s) {
+                             __bf16* out, hipStream_t s, int planes) {
     if (R <= 0 || K <= 0) return OMB_OK;
     const int64_t rtiles = ceil_div(R, TM), nkt = ceil_div(K, TK);
     const int64_t kgroups = ceil_div(nkt, SPLIT_SL);
@@ -376,8 +380,12 @@ omb_status launch_split_pack(int64_t R, int64_t K, const float* X, int64_t rs, i
         return OMB_EUNSUPPORTED;
     }
     const int k_major = cs == 1 ? 1 : 0;
-    hipLaunchKernelGGL(split_pack_kernel<false>, dim3((unsigned)rtiles, (unsigned)kgroups),
-                       dim3(NT), 0, s, X, R, K, rs, cs, nkt, k_major, out, nullptr, nullptr);
+    if (planes == 2)
+        hipLaunchKernelGGL((split_pack_kernel<false, 2>), dim3((unsigned)rtiles, (unsigned)kgroups),
+                           dim3(NT), 0, s, X, R, K, rs, cs, nkt, k_major, out, nullptr, nullptr);
+    else
+        hipLaunchKernelGGL((split_pack_kernel<false, 3>), dim3((unsigned)rtiles, (unsigned)kgroups),
+                           dim3(NT), 0, s, X, R, K, rs, cs, nkt, k_major, out, nullptr, nullptr);
     OMB_HIP_TRY(hipGetLastError());
     return OMB_OK;
 }
@@ -409,7 +417,7 @@ omb_status launch_split16_pack(int64_t R, int64_t K, const float* X, int64_t rs,
     }
     OMB_HIP_TRY(hipGetLastError());
     const int k_major = cs == 1 ? 1 : 0;
-    hipLaunchKernelGGL(split_pack_kernel<true>, dim3((unsigned)rtiles, (unsigned)kgroups),
+    hipLaunchKernelGGL((split_pack_kernel<true, 2>), dim3((unsigned)rtiles, (unsigned)kgroups),
                        dim3(NT), 0, s, X, R, K, rs, cs, nkt, k_major, out, amax, exps);
     OMB_HIP_TRY(hipGetLastError());
     return OMB_OK;
@@ -427,7 +435,8 @@ int64_t sgemm_split_splits(int64_t M, int64_t N, int64_t K, int64_t* kchunk) {
 
 omb_status launch_sgemm_packed(int64_t M, int64_t N, int64_t K, float alpha, const __bf16* ap,
                                const __bf16* bp, float beta, float* C, int64_t ldc, int nprod,
-                               hipStream_t s, float* part, const int* ea, const int* eb) {
+                               hipStream_t s, float* part, const int* ea, const int* eb,
+                               int planes) {
     const int64_t tiles_m = ceil_div(M, TM), tiles_n = ceil_div(N, TN);
     if (tiles_m * tiles_n > 0x7fffffffLL) {
         set_error("sgemm split: too many tiles for one launch");
@@ -451,6 +460,8 @@ omb_status launch_sgemm_packed(int64_t M, int64_t N, int64_t K, float alpha, con
     dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)splits), block(NT);
     if (ea)
         hipLaunchKernelGGL((sgemm_bf16x3_kernel<3, f16x8, 2>), grid, block, 0, s, a);
+    else if (nprod == 3 && planes == 2)
+        hipLaunchKernelGGL((sgemm_bf16x3_kernel<3, bf16x8, 2>), grid, block, 0, s, a);
     else if (nprod == 3)
         hipLaunchKernelGGL((sgemm_bf16x3_kernel<3, bf16x8, 3>), grid, block, 0, s, a);
     else
@@ -464,8 +475,12 @@ omb_status launch_sgemm_split(int64_t M, int64_t N, int64_t K, float alpha,
                               const float* A, int64_t rsA, int64_t csA,
                               const float* B, int64_t rsB, int64_t csB,
                               float beta, float* C, int64_t ldc, int nprod, hipStream_t s) {
-    const size_t a_elems = (size_t)split_packed_elems(M, K);
-    const size_t b_elems = (size_t)split_packed_elems(N, K);
+    // the 3-product form packs only the two planes it reads
+    const int planes = nprod == 3 ? 2 : 3;
+    const size_t a_elems = (size_t)(planes == 2 ? split16_packed_elems(M, K)
+                                                : split_packed_elems(M, K));
+    const size_t b_elems = (size_t)(planes == 2 ? split16_packed_elems(N, K)
+                                                : split_packed_elems(N, K));
     // one workspace request for the packed operands and any split partials
     const size_t packed = round_up((int64_t)((a_elems + b_elems) * sizeof(__bf16)), 256);
     int64_t kchunk = K;
@@ -477,9 +492,10 @@ omb_status launch_sgemm_split(int64_t M, int64_t N, int64_t K, float alpha,
     __bf16* bp = ap + a_elems;
     float* part = partial ? reinterpret_cast<float*>(static_cast<char*>(ws->buf) + packed) : nullptr;
     // A as rows m, columns k; B^T as rows n, columns k (element (n,k) = B(k,n)).
-    OMB_TRY(launch_split_pack(M, K, A, rsA, csA, ap, s));
-    OMB_TRY(launch_split_pack(N, K, B, csB, rsB, bp, s));
-    return launch_sgemm_packed(M, N, K, alpha, ap, bp, beta, C, ldc, nprod, s, part);
+    OMB_TRY(launch_split_pack(M, K, A, rsA, csA, ap, s, planes));
+    OMB_TRY(launch_split_pack(N, K, B, csB, rsB, bp, s, planes));
+    return launch_sgemm_packed(M, N, K, alpha, ap, bp, beta, C, ldc, nprod, s, part, nullptr,
+                               nullptr, planes);
 }
 
 // Scaled FP16 split: the exponents (M + N ints) and row maxima (M + N
