@@ -45,7 +45,7 @@ typedef void* omb_stream_t; /* hipStream_t */
  * carried to within max(2^-22 |x|, 2^-39 amax), amax the largest magnitude
  * of its row (A) or column (B): normwise as accurate as plain FP32, while an
  * element far below amax keeps only that absolute accuracy.
- * omb_sgemm only (not the grouped or serving entry points). */
+ * omb_sgemm and the serving loop (not the grouped plans). */
 #define OMB_MODE_SPLIT_FP16 3
 
 /* Last error message of the calling thread ("" when none). */

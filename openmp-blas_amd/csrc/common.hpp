@@ -117,6 +117,13 @@ omb_status launch_sgemm_split16(int64_t M, int64_t N, int64_t K, float alpha,
                                 const float* A, int64_t rsA, int64_t csA,
                                 const float* B, int64_t rsB, int64_t csB,
                                 float beta, float* C, int64_t ldc, hipStream_t s);
+// The same on a caller-supplied workspace of split16_ws_bytes(M, N, K) bytes
+// (256-byte aligned).
+size_t split16_ws_bytes(int64_t M, int64_t N, int64_t K);
+omb_status launch_sgemm_split16_ws(int64_t M, int64_t N, int64_t K, float alpha,
+                                   const float* A, int64_t rsA, int64_t csA,
+                                   const float* B, int64_t rsB, int64_t csB,
+                                   float beta, float* C, int64_t ldc, char* ws, hipStream_t s);
 
 // Scale C (row-major) by beta (K == 0 case).
 omb_status launch_scale(int64_t M, int64_t N, float beta, float* C, int64_t ldc, hipStream_t s);

@@ -108,6 +108,7 @@ size_t need_bytes(const omb_request& r) {
                 const int64_t splits = sgemm_f32_splits(r.m, r.n, r.k, &kc);
                 return splits > 1 ? (size_t)(splits * r.m * r.n) * sizeof(float) : 0;
             }
+            if (r.mode == OMB_MODE_SPLIT_FP16) return split16_ws_bytes(r.m, r.n, r.k);
             int64_t kc = 0;
             const int64_t splits = sgemm_split_splits(r.m, r.n, r.k, &kc);
             return packed_bytes(r) +
@@ -127,17 +128,12 @@ double flop_of(const omb_request& r) {
 }
 
 omb_status check_request(const omb_request& r, int i) {
-    if (r.kind == OMB_REQ_GEMM && r.mode == OMB_MODE_SPLIT_FP16) {
-        set_error("omb_serve_run: request " + std::to_string(i) +
-                  ": the scaled FP16 split runs through omb_sgemm only");
-        return OMB_EUNSUPPORTED;
-    }
     bool ok = true;
     switch (r.kind) {
         case OMB_REQ_GEMM:
             ok = r.m >= 0 && r.n >= 0 && r.k >= 0 &&
                  (r.mode == OMB_MODE_PLAIN_F32 || r.mode == OMB_MODE_SPLIT_BF16 ||
-                  r.mode == OMB_MODE_SPLIT_BF16_P3) &&
+                  r.mode == OMB_MODE_SPLIT_BF16_P3 || r.mode == OMB_MODE_SPLIT_FP16) &&
                  (r.m == 0 || r.n == 0 || (r.c && (r.k == 0 || (r.a && r.b))));
             break;
         case OMB_REQ_GEMV:
@@ -162,6 +158,9 @@ omb_status launch_request(const omb_request& r, char* ws, hipStream_t s) {
             if (r.mode == OMB_MODE_PLAIN_F32)
                 return launch_sgemm_f32(r.m, r.n, r.k, 1.f, r.a, r.k, true, r.b, r.n, true, 0.f, r.c,
                                         r.n, s, reinterpret_cast<float*>(ws));
+            if (r.mode == OMB_MODE_SPLIT_FP16)
+                return launch_sgemm_split16_ws(r.m, r.n, r.k, 1.f, r.a, r.k, 1, r.b, r.n, 1, 0.f,
+                                               r.c, r.n, ws, s);
             __bf16* ap = reinterpret_cast<__bf16*>(ws);
             __bf16* bp = ap + split_packed_elems(r.m, r.k);
             OMB_TRY(launch_split_pack(r.m, r.k, r.a, r.k, 1, ap, s));

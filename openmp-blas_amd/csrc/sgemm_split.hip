@@ -504,31 +504,48 @@ size_t split16_extra_bytes(int64_t M, int64_t N) {
     return (size_t)round_up((int64_t)((M + N) * (sizeof(int) + sizeof(unsigned))), 256);
 }
 
+// Workspace layout of one scaled split-FP16 product: the 2-plane packed A
+// and B^T, the exponents and maxima, then any split partials.
+size_t split16_ws_bytes(int64_t M, int64_t N, int64_t K) {
+    const size_t packed = round_up(
+        (split16_packed_elems(M, K) + split16_packed_elems(N, K)) * (int64_t)sizeof(__bf16), 256);
+    int64_t kchunk = K;
+    const int64_t splits = sgemm_split_splits(M, N, K, &kchunk);
+    return packed + split16_extra_bytes(M, N) +
+           (splits > 1 ? (size_t)(splits * M * N) * sizeof(float) : 0);
+}
+
+omb_status launch_sgemm_split16_ws(int64_t M, int64_t N, int64_t K, float alpha,
+                                   const float* A, int64_t rsA, int64_t csA,
+                                   const float* B, int64_t rsB, int64_t csB,
+                                   float beta, float* C, int64_t ldc, char* ws, hipStream_t s) {
+    const size_t a_elems = (size_t)split16_packed_elems(M, K);
+    const size_t b_elems = (size_t)split16_packed_elems(N, K);
+    const size_t packed = round_up((int64_t)((a_elems + b_elems) * sizeof(__bf16)), 256);
+    int64_t kchunk = K;
+    const int64_t splits = sgemm_split_splits(M, N, K, &kchunk);
+    __bf16* ap = reinterpret_cast<__bf16*>(ws);
+    __bf16* bp = ap + a_elems;
+    int* ea = reinterpret_cast<int*>(ws + packed);
+    int* eb = ea + M;
+    unsigned* amax_a = reinterpret_cast<unsigned*>(eb + N);
+    unsigned* amax_b = amax_a + M;
+    float* part =
+        splits > 1 ? reinterpret_cast<float*>(ws + packed + split16_extra_bytes(M, N)) : nullptr;
+    // A scaled by rows, B^T (rows n) by rows, i.e. B by columns.
+    OMB_TRY(launch_split16_pack(M, K, A, rsA, csA, ap, amax_a, ea, s));
+    OMB_TRY(launch_split16_pack(N, K, B, csB, rsB, bp, amax_b, eb, s));
+    return launch_sgemm_packed(M, N, K, alpha, ap, bp, beta, C, ldc, 3, s, part, ea, eb, 2);
+}
+
 omb_status launch_sgemm_split16(int64_t M, int64_t N, int64_t K, float alpha,
                                 const float* A, int64_t rsA, int64_t csA,
                                 const float* B, int64_t rsB, int64_t csB,
                                 float beta, float* C, int64_t ldc, hipStream_t s) {
-    const size_t a_elems = (size_t)split16_packed_elems(M, K);
-    const size_t b_elems = (size_t)split16_packed_elems(N, K);
-    const size_t packed = round_up((int64_t)((a_elems + b_elems) * sizeof(__bf16)), 256);
-    const size_t extra = split16_extra_bytes(M, N);
-    int64_t kchunk = K;
-    const int64_t splits = sgemm_split_splits(M, N, K, &kchunk);
-    const size_t partial = splits > 1 ? (size_t)(splits * M * N) * sizeof(float) : 0;
     Workspace* ws = nullptr;
-    OMB_TRY(get_workspace(s, packed + extra + partial, &ws));
-    char* base = static_cast<char*>(ws->buf);
-    __bf16* ap = reinterpret_cast<__bf16*>(base);
-    __bf16* bp = ap + a_elems;
-    int* ea = reinterpret_cast<int*>(base + packed);
-    int* eb = ea + M;
-    unsigned* amax_a = reinterpret_cast<unsigned*>(eb + N);
-    unsigned* amax_b = amax_a + M;
-    float* part = partial ? reinterpret_cast<float*>(base + packed + extra) : nullptr;
-    // A scaled by rows, B^T (rows n) by rows, i.e. B by columns.
-    OMB_TRY(launch_split16_pack(M, K, A, rsA, csA, ap, amax_a, ea, s));
-    OMB_TRY(launch_split16_pack(N, K, B, csB, rsB, bp, amax_b, eb, s));
-    return launch_sgemm_packed(M, N, K, alpha, ap, bp, beta, C, ldc, 3, s, part, ea, eb);
+    OMB_TRY(get_workspace(s, split16_ws_bytes(M, N, K), &ws));
+    return launch_sgemm_split16_ws(M, N, K, alpha, A, rsA, csA, B, rsB, csB, beta, C, ldc,
+                                   static_cast<char*>(ws->buf), s);
 }
 
 }  // namespace omb

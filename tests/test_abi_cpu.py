@@ -144,7 +144,6 @@ def test_serve_run_rejects_bad_arguments(ompblas):
     assert run([ok], cfg=Cfg(2, -5, 0, 1)) == -1                     # negative budget
     assert run([R(7, 0, 1, 1, 1, 0.0, 16, 16, 16, 0)]) == -1         # unknown kind
     assert run([R(ompblas.REQ_GEMM, 9, 4, 4, 4, 0.0, 16, 16, 16, 0)]) == -1   # bad mode
-    assert run([R(ompblas.REQ_GEMM, ompblas.SPLIT_FP16, 4, 4, 4, 0.0, 16, 16, 16, 0)]) == -2
     assert run([R(ompblas.REQ_GEMM, 1, 4, 4, 4, 0.0, 16, None, 16, 0)]) == -1  # missing B
     assert run([R(ompblas.REQ_GEMV, 0, 4, 4, 0, 0.0, 16, 16, None, 0)]) == -1  # missing y
     assert run([ok, R(ompblas.REQ_DOT, 0, 0, 8, 0, -1.0, 16, 16, 16, 0)]) == -1  # arrival < 0
@@ -165,6 +164,10 @@ def test_serve_request_bytes(ompblas):
     plain = ompblas.serve_request_bytes(R(ompblas.REQ_GEMM, 0, 300, 200, 500, 0.0, 16, 16, 16, 0))
     assert plain == 4 * 300 * 200 * 4
     assert ompblas.serve_request_bytes(R(ompblas.REQ_GEMM, 0, 300, 200, 100, 0.0, 16, 16, 16, 0)) == 0
+    # scaled FP16: 2-plane packed operands, then exponents and maxima of every line
+    f16 = ompblas.serve_request_bytes(R(ompblas.REQ_GEMM, 3, 300, 200, 260, 0.0, 16, 16, 16, 0))
+    assert f16 % 256 == 0 and f16 >= 2 * (300 + 200) * 260 * 2 + (300 + 200) * 8
+    assert f16 < split
     assert ompblas.serve_request_bytes(R(ompblas.REQ_GEMV, 0, 30, 20, 0, 0.0, 16, 16, 16, 0)) == 0
     assert ompblas.serve_request_bytes(R(ompblas.REQ_DOT, 0, 0, 5000, 0, 0.0, 16, 16, 16, 0)) > 0
     with pytest.raises(ompblas.OmbError):

@@ -70,7 +70,7 @@ def small_trace(ompblas, count, rate, seed):
                                  dot=(1 << 10, 1 << 20))
 
 
-@pytest.mark.parametrize("mode", [1, 0])
+@pytest.mark.parametrize("mode", [1, 0, 3])
 def test_mix_equals_solo_bitwise(dev, ompblas, mode):
     """High arrival rate, so requests overlap on the device; each output is
     bitwise equal to its solo run."""
@@ -129,7 +129,7 @@ def test_small_budget_makes_requests_wait(dev, ompblas):
         assert torch.equal(mixed[i], tr.solo(i)), i
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1, 3])
 def test_split_k_requests_equal_solo(dev, ompblas, mode):
     """GEMMs with few output tiles and long K take the split-K path, whose
     partial sums live in the request's workspace from the arena; overlapped
