@@ -45,7 +45,7 @@ typedef void* omb_stream_t; /* hipStream_t */
  * carried to within max(2^-22 |x|, 2^-39 amax), amax the largest magnitude
  * of its row (A) or column (B): normwise as accurate as plain FP32, while an
  * element far below amax keeps only that absolute accuracy.
- * omb_sgemm and the serving loop (not the grouped plans). */
+ * Every GEMM entry point (omb_sgemm, grouped plans, the serving loop). */
 #define OMB_MODE_SPLIT_FP16 3
 
 /* Last error message of the calling thread ("" when none). */
@@ -181,8 +181,9 @@ typedef struct omb_group_plan omb_group_plan;
 omb_status omb_group_plan_create(const omb_gemm_desc* gemms, int32_t n_gemm,
                                  const omb_gemv_desc* gemvs, int32_t n_gemv,
                                  int32_t mode, omb_stream_t stream, omb_group_plan** plan);
-/* Replay the captured graph (one persistent kernel, plus split pre-passes in
- * split-BF16 mode). */
+/* Replay the captured graph: one persistent kernel, preceded in the scaled
+ * split-FP16 mode by one kernel that computes every row and column exponent
+ * (plan-owned memory: launches of one plan must not overlap). */
 omb_status omb_group_plan_launch(omb_group_plan* plan, omb_stream_t stream);
 /* Run the same work without the graph (direct kernel launches). */
 omb_status omb_group_plan_launch_direct(omb_group_plan* plan, omb_stream_t stream);

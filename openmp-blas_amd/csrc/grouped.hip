@@ -11,9 +11,10 @@
 //
 // Every tile class keeps the per-element arithmetic of the standalone kernels
 // (plain: sequential k FMAs folded every 128 k; split-BF16: the same MFMA
-// slices, product order and 256-k folds; GEMV: the same lane order and
-// butterfly), so a grouped result is bitwise the one omb_sgemm / omb_sgemv
-// gives for that instance alone -- whatever else is in the batch.
+// slices, product order and 256-k folds; scaled split-FP16: the same line
+// exponents, pieces and products; GEMV: the same lane order and butterfly),
+// so a grouped result is bitwise the one omb_sgemm / omb_sgemv gives for
+// that instance alone -- whatever else is in the batch.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -41,6 +42,15 @@ struct DGemm {
     float* C;
     int64_t ldc;
     float alpha, beta;
+    int* ea;  // scaled FP16 split: exponent of every row of A, every column of B
+    int* eb;
+};
+
+// Exponent work of the scaled FP16 split: rows start .. start+3 of A (one
+// wavefront each) or columns start .. start+255 of B (one thread each).
+struct ExpItem {
+    int inst, col;
+    int64_t start;
 };
 
 struct DGemv {
@@ -69,6 +79,7 @@ struct PArgs {
     const DGemv* gemvs;
     unsigned* counters;  // [0] next dynamic chunk, [1] workgroups done
     int nprod;
+    int f16;      // split tiles take the scaled FP16 pieces (3 products)
     int chunk;    // consecutive items per chunk
     int nchunks;  // ceil(nitems / chunk)
 };
@@ -299,6 +310,113 @@ __device__ __forceinline__ void tile_split(const DGemm& g, int tm, int tn, uint4
     store_tile(outer, m0 + wm * 32, n0 + wn * 32, lane, g.C, g.ldc, M, N, g.alpha, g.beta);
 }
 
+// --------------------------------------------------- scaled FP16 split --
+// Line exponents e = 14 - ilogb(max |x|) (0 for a zero, infinite or NaN
+// line), one per row of A and column of B.  The maximum is taken over the
+// bits of |x| (unsigned order is the numeric order for non-negative floats;
+// NaN's bits exceed infinity's) and is order-free, so the exponents are the
+// ones omb_sgemm's maxima passes give.
+__global__ __launch_bounds__(NT) void grouped_exps_kernel(const ExpItem* __restrict__ items,
+                                                          const DGemm* __restrict__ gemms) {
+    const ExpItem w = items[blockIdx.x];
+    const DGemm& g = gemms[w.inst];
+    if (w.col == 0) {
+        const int lane = threadIdx.x & 63;
+        const int64_t m = w.start + (threadIdx.x >> 6);
+        if (m >= g.M) return;  // uniform across the wavefront
+        unsigned mx = 0;
+        for (int64_t k = lane; k < g.K; k += 64) {
+            const unsigned b = __float_as_uint(g.A[m * g.lda + k]) & 0x7fffffffu;
+            mx = b > mx ? b : mx;
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned o = (unsigned)__shfl_xor((int)mx, off);
+            mx = o > mx ? o : mx;
+        }
+        if (lane == 0) g.ea[m] = split::line_exp(__uint_as_float(mx));
+    } else {
+        const int64_t n = w.start + threadIdx.x;
+        if (n >= g.N) return;
+        unsigned mx = 0;
+#pragma unroll 8
+        for (int64_t k = 0; k < g.K; ++k) {
+            const unsigned b = __float_as_uint(g.B[k * g.ldb + n]) & 0x7fffffffu;
+            mx = b > mx ? b : mx;
+        }
+        g.eb[n] = split::line_exp(__uint_as_float(mx));
+    }
+}
+
+// The split tile with the scaled FP16 pieces: every element is scaled by
+// its row (A) or column (B) exponent and split into two FP16 pieces as
+// omb_sgemm's packing pass does; the 3 leading products (a0 b1, a1 b0,
+// a0 b0) run on the FP16 MFMA, and the sums are scaled back when stored.
+__device__ __forceinline__ void tile_split16(const DGemm& g, int tm, int tn, uint4* smem) {
+    using namespace split;
+    constexpr int P = 64;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int frow = lane & 31, fh = lane >> 5;
+    const int64_t m0 = (int64_t)tm * P, n0 = (int64_t)tn * P;
+    const int64_t M = g.M, N = g.N, K = g.K;
+    const int ra = tid >> 2, qa = tid & 3;
+    const int rb = tid & 63, qb = tid >> 6;
+    uint2* l2 = reinterpret_cast<uint2*>(smem);
+    const int ex_a = m0 + ra < M ? g.ea[m0 + ra] : 0;
+    const int ex_b = n0 + rb < N ? g.eb[n0 + rb] : 0;
+
+    f32x16 inner[1][1], outer[1][1];
+    inner[0][0] = outer[0][0] = f32x16{};
+    const int64_t nkt = ceil_div(K, TK);
+    for (int64_t kt = 0; kt < nkt; ++kt) {
+        const int64_t k0 = kt * TK;
+        float va[4], vb[4];
+        {
+            const int64_t m = m0 + ra, k = k0 + 4 * qa;
+            if (m < M && k + 3 < K) {
+                const float4 t = ld4u(g.A + m * g.lda + k);
+                va[0] = t.x; va[1] = t.y; va[2] = t.z; va[3] = t.w;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) va[e] = (m < M && k + e < K) ? g.A[m * g.lda + k + e] : 0.f;
+            }
+        }
+        {
+            const int64_t n = n0 + rb, k = k0 + 4 * qb;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) vb[e] = (n < N && k + e < K) ? g.B[(k + e) * g.ldb + n] : 0.f;
+        }
+        __syncthreads();  // previous slice consumed
+        {
+            typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+            u16x4 pa[2], pb[2];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                _Float16 h0, h1;
+                split2_f16(ldexpf(va[e], ex_a), h0, h1);
+                pa[0][e] = __builtin_bit_cast(unsigned short, h0);
+                pa[1][e] = __builtin_bit_cast(unsigned short, h1);
+                split2_f16(ldexpf(vb[e], ex_b), h0, h1);
+                pb[0][e] = __builtin_bit_cast(unsigned short, h0);
+                pb[1][e] = __builtin_bit_cast(unsigned short, h1);
+            }
+#pragma unroll
+            for (int pl = 0; pl < 2; ++pl) {
+                l2[2 * (pl * 2 * P + lds_chunk(ra, qa >> 1)) + (qa & 1)] =
+                    __builtin_bit_cast(uint2, pa[pl]);
+                l2[2 * (2 * 2 * P + pl * 2 * P + lds_chunk(rb, qb >> 1)) + (qb & 1)] =
+                    __builtin_bit_cast(uint2, pb[pl]);
+            }
+        }
+        __syncthreads();
+        mfma_slice<3, 1, 1, P, P, f16x8, 2>(smem, wm * 32, wn * 32, frow, fh, inner);
+        if ((kt + 1) % kFoldSlices == 0 || kt + 1 == nkt) fold(inner, outer);
+    }
+    store_tile(outer, m0 + wm * 32, n0 + wn * 32, lane, g.C, g.ldc, M, N, g.alpha, g.beta, g.ea,
+               g.eb);
+}
+
 // ------------------------------------------------------------- GEMV item --
 __device__ __forceinline__ void gemv_item(const DGemv& v, int chunk) {
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -342,7 +460,9 @@ __global__ __launch_bounds__(NT, 2) void grouped_kernel(PArgs p) {
                     if (BIG) tile_f32<128, 64, 8, 4>(p.gemms[w.inst], w.tm, w.tn, smem);
                     break;
                 case CSPLIT:
-                    if (p.nprod == 3)
+                    if (p.f16)
+                        tile_split16(p.gemms[w.inst], w.tm, w.tn, reinterpret_cast<uint4*>(smem));
+                    else if (p.nprod == 3)
                         tile_split<3>(p.gemms[w.inst], w.tm, w.tn, reinterpret_cast<uint4*>(smem));
                     else
                         tile_split<6>(p.gemms[w.inst], w.tm, w.tn, reinterpret_cast<uint4*>(smem));
@@ -419,6 +539,9 @@ struct omb_group_plan {
     omb::DGemv* d_gemvs = nullptr;
     omb::Item* d_items = nullptr;
     unsigned* d_counters = nullptr;
+    int nexp = 0;                      // scaled FP16 split: exponent work items
+    omb::ExpItem* d_exp_items = nullptr;
+    int* d_exps = nullptr;             // every GEMM's row and column exponents
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::string text;
@@ -483,13 +606,21 @@ void free_plan(omb_group_plan* p) {
     if (p->d_gemvs) (void)hipFree(p->d_gemvs);
     if (p->d_items) (void)hipFree(p->d_items);
     if (p->d_counters) (void)hipFree(p->d_counters);
+    if (p->d_exp_items) (void)hipFree(p->d_exp_items);
+    if (p->d_exps) (void)hipFree(p->d_exps);
     delete p;
 }
 
 omb_status launch_plan(omb_group_plan* p, hipStream_t s) {
     if (p->nitems == 0) return OMB_OK;
+    if (p->nexp > 0) {  // exponents first (same stream: the tiles read them)
+        hipLaunchKernelGGL(grouped_exps_kernel, dim3(p->nexp), dim3(NT), 0, s, p->d_exp_items,
+                           p->d_gemms);
+        OMB_HIP_TRY(hipGetLastError());
+    }
+    const bool f16 = p->mode == OMB_MODE_SPLIT_FP16;
     PArgs a{p->d_items, p->nitems, p->d_gemms, p->d_gemvs, p->d_counters,
-            p->mode == OMB_MODE_SPLIT_BF16_P3 ? 3 : 6, p->chunk,
+            p->mode == OMB_MODE_SPLIT_BF16_P3 || f16 ? 3 : 6, f16 ? 1 : 0, p->chunk,
             (int)ceil_div(p->nitems, p->chunk)};
     if (p->big)
         hipLaunchKernelGGL(grouped_kernel<true>, dim3(p->grid), dim3(NT), 0, s, a);
@@ -504,12 +635,8 @@ omb_status build_plan(const omb_gemm_desc* gemms, int32_t n_gemm, const omb_gemv
                       int32_t n_gemv, int32_t mode, int force_cls, bool capture,
                       omb_group_plan** out) {
     *out = nullptr;
-    if (mode == OMB_MODE_SPLIT_FP16) {
-        set_error("group plan: the scaled FP16 split runs through omb_sgemm only");
-        return OMB_EUNSUPPORTED;
-    }
     if (mode != OMB_MODE_PLAIN_F32 && mode != OMB_MODE_SPLIT_BF16 &&
-        mode != OMB_MODE_SPLIT_BF16_P3) {
+        mode != OMB_MODE_SPLIT_BF16_P3 && mode != OMB_MODE_SPLIT_FP16) {
         set_error("group plan: unknown mode " + std::to_string(mode));
         return OMB_EINVAL;
     }
@@ -523,7 +650,8 @@ omb_status build_plan(const omb_gemm_desc* gemms, int32_t n_gemm, const omb_gemv
     std::map<std::string, int> chosen;
     for (int i = 0; i < n_gemm; ++i) {
         const omb_gemm_desc& d = gemms[i];
-        g[i] = DGemm{d.M, d.N, d.K, d.A, d.lda, d.B, d.ldb, d.C, d.ldc, d.alpha, d.beta};
+        g[i] = DGemm{d.M, d.N, d.K, d.A, d.lda, d.B, d.ldb, d.C, d.ldc, d.alpha, d.beta,
+                     nullptr, nullptr};
         if (d.M == 0 || d.N == 0) continue;
         const int c = force_cls >= 0 ? force_cls : choose_class(mode, d);
         const int64_t tmn = ceil_div(d.M, kTileM[c]), tnn = ceil_div(d.N, kTileN[c]);
@@ -556,6 +684,38 @@ omb_status build_plan(const omb_gemm_desc* gemms, int32_t n_gemm, const omb_gemv
     p->nitems = (int)sorted.size();
 
     omb_status st = OMB_OK;
+    if (mode == OMB_MODE_SPLIT_FP16) {
+        // one exponent per row of A and column of B, in a plan-owned buffer
+        std::vector<ExpItem> ex;
+        int64_t lines = 0;
+        for (int i = 0; i < n_gemm; ++i)
+            if (g[i].M > 0 && g[i].N > 0) lines += g[i].M + g[i].N;
+        if (lines > 0 && hipMalloc(&p->d_exps, lines * sizeof(int)) != hipSuccess) {
+            (void)hipGetLastError();
+            free_plan(p);
+            set_error("group plan: exponent allocation failed");
+            return OMB_ENOMEM;
+        }
+        int64_t off = 0;
+        for (int i = 0; i < n_gemm; ++i) {
+            if (g[i].M == 0 || g[i].N == 0) continue;
+            g[i].ea = p->d_exps + off;
+            g[i].eb = p->d_exps + off + g[i].M;
+            off += g[i].M + g[i].N;
+            for (int64_t r = 0; r < g[i].M; r += NT / 64) ex.push_back(ExpItem{i, 0, r});
+            for (int64_t c = 0; c < g[i].N; c += NT) ex.push_back(ExpItem{i, 1, c});
+        }
+        if (ex.size() > 0x7fffffff) {
+            free_plan(p);
+            set_error("group plan: too many exponent items");
+            return OMB_EUNSUPPORTED;
+        }
+        p->nexp = (int)ex.size();
+        if ((st = upload(&p->d_exp_items, ex)) != OMB_OK) {
+            free_plan(p);
+            return st;
+        }
+    }
     if ((st = upload(&p->d_gemms, g)) != OMB_OK || (st = upload(&p->d_gemvs, v)) != OMB_OK ||
         (st = upload(&p->d_items, sorted)) != OMB_OK) {
         free_plan(p);
@@ -719,12 +879,8 @@ extern "C" omb_status omb_autotune_group(const omb_gemm_desc* gemms, int32_t n_g
                                          const char* table_path, omb_stream_t stream) {
     clear_error();
     OMB_TRY(validate(gemms, n_gemm, nullptr, 0));
-    if (mode == OMB_MODE_SPLIT_FP16) {
-        set_error("omb_autotune_group: the scaled FP16 split runs through omb_sgemm only");
-        return OMB_EUNSUPPORTED;
-    }
     if (mode != OMB_MODE_PLAIN_F32 && mode != OMB_MODE_SPLIT_BF16 &&
-        mode != OMB_MODE_SPLIT_BF16_P3) {
+        mode != OMB_MODE_SPLIT_BF16_P3 && mode != OMB_MODE_SPLIT_FP16) {
         set_error("omb_autotune_group: unknown mode");
         return OMB_EINVAL;
     }

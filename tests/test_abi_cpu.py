@@ -115,10 +115,7 @@ def test_group_plan_rejects_bad_descriptors(ompblas):
     badv = (ompblas.GemvDesc * 1)(ompblas.GemvDesc(4, 8, 16, 4, 16, 16, 1.0, 0.0))       # lda < N
     assert lib.omb_group_plan_create(None, 0, badv, 1, 0, None, ctypes.byref(h)) == -1
     assert lib.omb_group_plan_create(None, 0, None, 0, 9, None, ctypes.byref(h)) == -1   # mode
-    # the scaled FP16 split is a direct-call mode
-    assert lib.omb_group_plan_create(None, 0, None, 0, ompblas.SPLIT_FP16, None,
-                                     ctypes.byref(h)) == -2
-    assert lib.omb_autotune_group(None, 0, ompblas.SPLIT_FP16, None, None) == -2
+    assert lib.omb_autotune_group(None, 0, 9, None, None) == -1                          # mode
     assert lib.omb_group_plan_create(None, -1, None, 0, 0, None, ctypes.byref(h)) == -1
     assert "group plan" in ompblas.last_error()
     assert lib.omb_autotune_load(b"/nonexistent/table.txt") == -1

@@ -77,7 +77,7 @@ def shapes(seed, n_gemm, n_gemv, lo=16, hi=512, vlo=64, vhi=4096):
     return [tuple(map(int, s)) for s in g], [tuple(map(int, s)) for s in v]
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_grouped_bitwise_equals_individual_calls(dev, ompblas, mode):
     gs, vs = shapes(3, 300, 120)
     b = Batch(dev, ompblas, gs, vs, seed=3)
@@ -98,13 +98,14 @@ def test_grouped_bitwise_equals_individual_calls(dev, ompblas, mode):
     plan.close()
 
 
-def test_grouped_beta_alpha_and_empty_instances(dev, ompblas):
+@pytest.mark.parametrize("mode", [0, 1, 3])
+def test_grouped_beta_alpha_and_empty_instances(dev, ompblas, mode):
     gs = [(33, 17, 0), (0, 5, 7), (5, 0, 7), (40, 70, 9), (1, 1, 1), (512, 3, 100)]
     vs = [(70, 0), (0, 10), (130, 257), (1, 1)]
     b = Batch(dev, ompblas, gs, vs, seed=5, beta=0.5)
-    want_c, want_y = b.eager(ompblas, 0)
+    want_c, want_y = b.eager(ompblas, mode)
     b.reset()
-    plan = ompblas.GroupPlan(b.gemm_descs(), b.gemv_descs(), 0)
+    plan = ompblas.GroupPlan(b.gemm_descs(), b.gemv_descs(), mode)
     plan.launch()
     torch.cuda.synchronize()
     assert torch.equal(b.gc, want_c) and torch.equal(b.vy, want_y)
@@ -181,3 +182,23 @@ def test_loaded_table_can_select_the_128x64_class(dev, ompblas, tmp_path):
         plan.close()
     finally:
         ompblas.autotune_load(str(back))
+
+
+def test_autotuner_scaled_fp16_mode(dev, ompblas, tmp_path):
+    """The split modes have one tile class: the autotuner records it for the
+    scaled FP16 split's buckets, and a plan built after loading the table is
+    bitwise equal to per-instance calls."""
+    gs, _ = shapes(17, 40, 0, hi=300)
+    b = Batch(dev, ompblas, gs, [], seed=17)
+    path = str(tmp_path / "fp16.txt")
+    ompblas.autotune_group(b.gemm_descs(), ompblas.SPLIT_FP16, path)
+    lines = [l.split() for l in open(path) if l.startswith("3 ")]   # the whole table is written
+    assert lines and all(l[4] == "bf16x3_64x64" for l in lines)
+    ompblas.autotune_load(path)
+    want_c, _ = b.eager(ompblas, ompblas.SPLIT_FP16)
+    b.reset()
+    plan = ompblas.GroupPlan(b.gemm_descs(), [], ompblas.SPLIT_FP16)
+    plan.launch()
+    torch.cuda.synchronize()
+    assert torch.equal(b.gc, want_c)
+    plan.close()

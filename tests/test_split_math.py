@@ -1,7 +1,7 @@
 """The arithmetic the split GEMM modes rest on, restated in numpy and pinned
 on the CPU (no GPU): the exactness of the 3-way BF16 split and the error
 bound of the power-of-two scaled 2-way FP16 split.  The GPU tests check the
-kernels' results; these check the claims those results are judged by."""
+kernels' results; these check the claims those results rest on."""
 import numpy as np
 import pytest
 
