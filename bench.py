@@ -32,6 +32,7 @@ METRIC = ("SGEMM TFLOP/s 8192^3 split-BF16 vs plain FP32 @≤2x FP32 err; "
           "GEMV GB/s; p99 ms")
 PEAK_HBM = 8.0e12
 PEAK_MFMA_BF16 = 2.5e15
+PEAK_MFMA_FP16 = 2.5e15
 PEAK_FP32 = 157.3e12
 
 
@@ -577,7 +578,8 @@ def grouped_leg(pkg, lib, torch, dev, stream, timer, steps, cpu_sample=True):
            "gemm_bytes": float(g_bytes.sum()), "gemv_bytes": float(v_bytes.sum()),
            "autotune_s": round(t_tune, 3)}
     for mode, name, rate in ((pkg.PLAIN_F32, "plain_f32", PEAK_FP32),
-                             (pkg.SPLIT_BF16, "split_bf16_p6", PEAK_MFMA_BF16 / 6)):
+                             (pkg.SPLIT_BF16, "split_bf16_p6", PEAK_MFMA_BF16 / 6),
+                             (pkg.SPLIT_FP16, "split_fp16_p3", PEAK_MFMA_FP16 / 3)):
         t_all, desc = time_plan(gemms, gemvs, mode)
         t_g, _ = time_plan(gemms, [], mode)
         t_v, _ = time_plan([], gemvs, mode)

@@ -46,8 +46,10 @@ struct DGemm {
     int* eb;
 };
 
-// Exponent work of the scaled FP16 split: rows start .. start+3 of A (one
-// wavefront each) or columns start .. start+255 of B (one thread each).
+// Exponent work of the scaled FP16 split: rows start .. start+31 of A (8
+// per wavefront) or columns start .. start+63 of B (a lane per column, the
+// k range shared by the 4 wavefronts).
+constexpr int kExpRows = 32, kExpCols = 64;
 struct ExpItem {
     int inst, col;
     int64_t start;
@@ -248,6 +250,34 @@ __device__ __forceinline__ void tile_f32(const DGemm& g, int tm, int tn, float* 
 // 64x64 tile on the MFMA cores: each 16-k slice of A (row-major) and B^T
 // (B row-major) is read as FP32, split into the three BF16 planes and
 // written into the swizzled LDS slice; 4 wavefronts own 32x32 each.
+// The 4 consecutive-k values of A (row m, from k ka) and of B (column n,
+// from k kb) that one loader thread stages per 16-k slice; elements out of
+// range read as 0.  Edge loads read a clamped in-range address and select,
+// so every load of the slice is in flight at once (A and B are non-null
+// whenever a tile has K > 0).
+__device__ __forceinline__ void load_split_slice(const float* A, int64_t lda, int64_t M,
+                                                 const float* B, int64_t ldb, int64_t N,
+                                                 int64_t K, int64_t m, int64_t ka, int64_t n,
+                                                 int64_t kb, float (&va)[4], float (&vb)[4]) {
+    if (m < M && ka + 3 < K) {
+        const float4 t = ld4u(A + m * lda + ka);
+        va[0] = t.x; va[1] = t.y; va[2] = t.z; va[3] = t.w;
+    } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const bool ok = m < M && ka + e < K;
+            const float t = A[ok ? m * lda + ka + e : 0];
+            va[e] = ok ? t : 0.f;
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const bool ok = n < N && kb + e < K;
+        const float t = B[ok ? (kb + e) * ldb + n : 0];
+        vb[e] = ok ? t : 0.f;
+    }
+}
+
 template <int NPROD>
 __device__ __forceinline__ void tile_split(const DGemm& g, int tm, int tn, uint4* smem) {
     using namespace split;
@@ -257,6 +287,9 @@ __device__ __forceinline__ void tile_split(const DGemm& g, int tm, int tn, uint4
     const int frow = lane & 31, fh = lane >> 5;
     const int64_t m0 = (int64_t)tm * P, n0 = (int64_t)tn * P;
     const int64_t M = g.M, N = g.N, K = g.K;
+    const float* const A = g.A;
+    const float* const B = g.B;
+    const int64_t lda = g.lda, ldb = g.ldb;
     // loader roles: A row ra, k quarter qa; B^T row rb (= column of B), k quarter qb
     const int ra = tid >> 2, qa = tid & 3;
     const int rb = tid & 63, qb = tid >> 6;
@@ -268,21 +301,7 @@ __device__ __forceinline__ void tile_split(const DGemm& g, int tm, int tn, uint4
     for (int64_t kt = 0; kt < nkt; ++kt) {
         const int64_t k0 = kt * TK;
         float va[4], vb[4];
-        {
-            const int64_t m = m0 + ra, k = k0 + 4 * qa;
-            if (m < M && k + 3 < K) {
-                const float4 t = ld4u(g.A + m * g.lda + k);
-                va[0] = t.x; va[1] = t.y; va[2] = t.z; va[3] = t.w;
-            } else {
-#pragma unroll
-                for (int e = 0; e < 4; ++e) va[e] = (m < M && k + e < K) ? g.A[m * g.lda + k + e] : 0.f;
-            }
-        }
-        {
-            const int64_t n = n0 + rb, k = k0 + 4 * qb;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) vb[e] = (n < N && k + e < K) ? g.B[(k + e) * g.ldb + n] : 0.f;
-        }
+        load_split_slice(A, lda, M, B, ldb, N, K, m0 + ra, k0 + 4 * qa, n0 + rb, k0 + 4 * qb, va, vb);
         __syncthreads();  // previous slice consumed
         {
             typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
@@ -318,33 +337,50 @@ __device__ __forceinline__ void tile_split(const DGemm& g, int tm, int tn, uint4
 // ones omb_sgemm's maxima passes give.
 __global__ __launch_bounds__(NT) void grouped_exps_kernel(const ExpItem* __restrict__ items,
                                                           const DGemm* __restrict__ gemms) {
+    __shared__ unsigned red[NT];
     const ExpItem w = items[blockIdx.x];
     const DGemm& g = gemms[w.inst];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int RW = kExpRows / (NT / 64);  // rows per wavefront
     if (w.col == 0) {
-        const int lane = threadIdx.x & 63;
-        const int64_t m = w.start + (threadIdx.x >> 6);
-        if (m >= g.M) return;  // uniform across the wavefront
-        unsigned mx = 0;
+        const int64_t m0 = w.start + wave * RW;
+        unsigned mx[RW];
+#pragma unroll
+        for (int r = 0; r < RW; ++r) mx[r] = 0;
         for (int64_t k = lane; k < g.K; k += 64) {
-            const unsigned b = __float_as_uint(g.A[m * g.lda + k]) & 0x7fffffffu;
-            mx = b > mx ? b : mx;
+#pragma unroll
+            for (int r = 0; r < RW; ++r)  // RW independent loads in flight
+                if (m0 + r < g.M) {
+                    const unsigned b = __float_as_uint(g.A[(m0 + r) * g.lda + k]) & 0x7fffffffu;
+                    mx[r] = b > mx[r] ? b : mx[r];
+                }
         }
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const unsigned o = (unsigned)__shfl_xor((int)mx, off);
-            mx = o > mx ? o : mx;
+        for (int r = 0; r < RW; ++r) {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const unsigned o = (unsigned)__shfl_xor((int)mx[r], off);
+                mx[r] = o > mx[r] ? o : mx[r];
+            }
+            if (lane == r && m0 + r < g.M) g.ea[m0 + r] = split::line_exp(__uint_as_float(mx[r]));
         }
-        if (lane == 0) g.ea[m] = split::line_exp(__uint_as_float(mx));
     } else {
-        const int64_t n = w.start + threadIdx.x;
-        if (n >= g.N) return;
+        const int64_t n = w.start + lane;
         unsigned mx = 0;
+        if (n < g.N) {
 #pragma unroll 8
-        for (int64_t k = 0; k < g.K; ++k) {
-            const unsigned b = __float_as_uint(g.B[k * g.ldb + n]) & 0x7fffffffu;
-            mx = b > mx ? b : mx;
+            for (int64_t k = wave; k < g.K; k += NT / 64) {
+                const unsigned b = __float_as_uint(g.B[k * g.ldb + n]) & 0x7fffffffu;
+                mx = b > mx ? b : mx;
+            }
         }
-        g.eb[n] = split::line_exp(__uint_as_float(mx));
+        red[threadIdx.x] = mx;
+        __syncthreads();
+        if (wave == 0 && n < g.N) {
+#pragma unroll
+            for (int q = 1; q < NT / 64; ++q) mx = red[q * 64 + lane] > mx ? red[q * 64 + lane] : mx;
+            g.eb[n] = split::line_exp(__uint_as_float(mx));
+        }
     }
 }
 
@@ -360,6 +396,9 @@ __device__ __forceinline__ void tile_split16(const DGemm& g, int tm, int tn, uin
     const int frow = lane & 31, fh = lane >> 5;
     const int64_t m0 = (int64_t)tm * P, n0 = (int64_t)tn * P;
     const int64_t M = g.M, N = g.N, K = g.K;
+    const float* const A = g.A;
+    const float* const B = g.B;
+    const int64_t lda = g.lda, ldb = g.ldb;
     const int ra = tid >> 2, qa = tid & 3;
     const int rb = tid & 63, qb = tid >> 6;
     uint2* l2 = reinterpret_cast<uint2*>(smem);
@@ -372,21 +411,7 @@ __device__ __forceinline__ void tile_split16(const DGemm& g, int tm, int tn, uin
     for (int64_t kt = 0; kt < nkt; ++kt) {
         const int64_t k0 = kt * TK;
         float va[4], vb[4];
-        {
-            const int64_t m = m0 + ra, k = k0 + 4 * qa;
-            if (m < M && k + 3 < K) {
-                const float4 t = ld4u(g.A + m * g.lda + k);
-                va[0] = t.x; va[1] = t.y; va[2] = t.z; va[3] = t.w;
-            } else {
-#pragma unroll
-                for (int e = 0; e < 4; ++e) va[e] = (m < M && k + e < K) ? g.A[m * g.lda + k + e] : 0.f;
-            }
-        }
-        {
-            const int64_t n = n0 + rb, k = k0 + 4 * qb;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) vb[e] = (n < N && k + e < K) ? g.B[(k + e) * g.ldb + n] : 0.f;
-        }
+        load_split_slice(A, lda, M, B, ldb, N, K, m0 + ra, k0 + 4 * qa, n0 + rb, k0 + 4 * qb, va, vb);
         __syncthreads();  // previous slice consumed
         {
             typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
@@ -433,8 +458,10 @@ __device__ __forceinline__ void gemv_item(const DGemv& v, int chunk) {
 // the split tile uses 12 KB.
 constexpr int kSmemFloats = 16 * (128 + 4) + 16 * (64 + 4) + 16;
 
+// The lite build is bounded to 3 workgroups per CU (at most 168 VGPRs), the
+// big one (128x64 tiles) to 2.
 template <bool BIG>
-__global__ __launch_bounds__(NT, 2) void grouped_kernel(PArgs p) {
+__global__ __launch_bounds__(NT, BIG ? 2 : 3) void grouped_kernel(PArgs p) {
     __shared__ __attribute__((aligned(16))) float smem[kSmemFloats];
     __shared__ int s_chunk;
     // Chunks of consecutive (cost-sorted) items: workgroup b starts with
@@ -702,8 +729,8 @@ omb_status build_plan(const omb_gemm_desc* gemms, int32_t n_gemm, const omb_gemv
             g[i].ea = p->d_exps + off;
             g[i].eb = p->d_exps + off + g[i].M;
             off += g[i].M + g[i].N;
-            for (int64_t r = 0; r < g[i].M; r += NT / 64) ex.push_back(ExpItem{i, 0, r});
-            for (int64_t c = 0; c < g[i].N; c += NT) ex.push_back(ExpItem{i, 1, c});
+            for (int64_t r = 0; r < g[i].M; r += kExpRows) ex.push_back(ExpItem{i, 0, r});
+            for (int64_t c = 0; c < g[i].N; c += kExpCols) ex.push_back(ExpItem{i, 1, c});
         }
         if (ex.size() > 0x7fffffff) {
             free_plan(p);
