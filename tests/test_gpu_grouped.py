@@ -202,3 +202,26 @@ def test_autotuner_scaled_fp16_mode(dev, ompblas, tmp_path):
     torch.cuda.synchronize()
     assert torch.equal(b.gc, want_c)
     plan.close()
+
+
+def test_grouped_fp16_exponents_see_every_element(dev, ompblas):
+    """Scaled FP16 mode with A and B elements scaled by powers of two from
+    2^-20 to 2^20 by position, so the maximum of each row of A and column of
+    B sits at a different k (ragged tails included) and decides that line's
+    exponent: grouped results equal per-instance calls bitwise only if the
+    exponent pass saw every element."""
+    gs = [(37, 29, 53), (64, 65, 7), (5, 130, 257), (129, 3, 300), (1, 1, 1), (70, 66, 4)]
+    b = Batch(dev, ompblas, gs, [], seed=21)
+    ar = lambda n: torch.arange(n, device=dev)
+    for i, (m, n, k) in enumerate(gs):
+        a = b.ga[b.oa[i]:b.oa[i] + m * k].view(m, k)
+        a.mul_(torch.exp2(((ar(m).view(-1, 1) * 7 + ar(k).view(1, -1) * 13 + i) % 41 - 20).float()))
+        bb = b.gb[b.ob[i]:b.ob[i] + k * n].view(k, n)
+        bb.mul_(torch.exp2(((ar(k).view(-1, 1) * 11 + ar(n).view(1, -1) * 5 + i) % 37 - 18).float()))
+    want_c, _ = b.eager(ompblas, ompblas.SPLIT_FP16)
+    b.reset()
+    plan = ompblas.GroupPlan(b.gemm_descs(), [], ompblas.SPLIT_FP16)
+    plan.launch()
+    torch.cuda.synchronize()
+    assert torch.equal(b.gc, want_c)
+    plan.close()
