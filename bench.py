@@ -104,7 +104,14 @@ def main():
                     help="serving leg: requests in the trace")
     ap.add_argument("--serve-slots", type=int, default=int(os.environ.get("OMB_SERVE_SLOTS", 8)),
                     help="serving leg: requests in flight at most (one stream each)")
+    ap.add_argument("--cpu-baseline-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-cores", type=int, default=0, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_baseline_child:   # the CPU leg's own process (no GPU runtime)
+        if not graft.built():
+            graft.build()
+        print(json.dumps(cpu_baseline(args.cpu_budget, args.cpu_cores or None)), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is a design, not a measured path)", file=sys.stderr)
@@ -122,6 +129,10 @@ def main():
         legs.discard("errors")
     if args.no_cpu:
         legs.discard("cpu")
+
+    # The CPU baseline runs first, in a child process, while this process has
+    # loaded neither torch nor its OpenMP runtime.
+    cpu = cpu_baseline_child(args.cpu_budget, host_cores()) if "cpu" in legs else None
 
     import torch
 
@@ -394,9 +405,8 @@ def main():
         sync()
 
     # ------------------------------------------------------- CPU baseline --
-    cpu, cpu_ref = None, None
+    cpu_ref = None
     if "cpu" in legs:
-        cpu = cpu_baseline(args.cpu_budget)
         cpu_ref = cpu_reference_legs(pkg, torch, dev, stream, refs)
 
     res = {
@@ -834,7 +844,49 @@ def cpu_reference_legs(pkg, torch, dev, stream, refs=None):
     return out
 
 
-def cpu_baseline(budget_s: float):
+def _getconf(name: str) -> int:
+    """One cache parameter as getconf(1) reports it (0 when unknown)."""
+    import subprocess
+    try:
+        return int(subprocess.run(["getconf", name], capture_output=True, text=True).stdout)
+    except (OSError, ValueError):
+        return 0
+
+
+def host_cache_keys(blk) -> dict:
+    """Flat record of the host's cache geometry as sysconf reports it and of
+    the blocking the oracle derives from it (MR/NR/KC/MC/NC, the mtv and dot
+    blocks).  The reference's Linux probe puts the L3 associativity in the
+    L2 slot (include/cache_manager.hpp:133-135); the blocking follows that."""
+    out = {}
+    for key, name in (("l1d", "LEVEL1_DCACHE"), ("l2", "LEVEL2_CACHE"), ("l3", "LEVEL3_CACHE")):
+        out[f"{key}_bytes"] = _getconf(name + "_SIZE")
+        out[f"{key}_assoc"] = _getconf(name + "_ASSOC")
+        out[f"{key}_line"] = _getconf(name + "_LINESIZE")
+    for k, v in blk.as_dict().items():
+        out[f"blk_{k}"] = v
+    return out
+
+
+def cpu_baseline_child(budget_s: float, cores: int):
+    """Run cpu_baseline in a child process that loads no GPU runtime and no
+    other OpenMP runtime, with the oracle's team placed one thread per core
+    (OMP_PLACES=cores, OMP_PROC_BIND=close, OMP_NUM_THREADS=cores).  Started
+    before this process imports torch."""
+    import subprocess
+    env = dict(os.environ, OMP_PLACES="cores", OMP_PROC_BIND="close",
+               OMP_NUM_THREADS=str(cores))
+    log(f"cpu baseline: child process, {cores} threads, OMP_PLACES=cores OMP_PROC_BIND=close")
+    out = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child",
+                          "--cpu-budget", str(budget_s), "--cpu-cores", str(cores)],
+                         capture_output=True, text=True, env=env)
+    sys.stderr.write(out.stderr)
+    if out.returncode != 0:
+        raise RuntimeError(f"cpu baseline child failed ({out.returncode})")
+    return json.loads(out.stdout.strip().splitlines()[-1])
+
+
+def cpu_baseline(budget_s: float, cores: int | None = None):
     """The oracle's mtm (the reference algorithm: BLIS 5-loop, MR=16, NR=6,
     KC/MC/NC from this host's caches, OpenMP over all visible cores) on the
     8192^3 FP32 product; mean of up to 4 timed calls after one warm-up
@@ -843,7 +895,7 @@ def cpu_baseline(budget_s: float):
     import helpers
 
     n = 8192
-    cores = host_cores()
+    cores = cores or host_cores()
     log(f"cpu baseline: oracle mtm 8192^3 on {cores} threads")
     a = helpers.rng_uniform(n * n, 1, 0).reshape(n, n)
     b = helpers.rng_uniform(n * n, 1, 1).reshape(n, n)
@@ -869,14 +921,17 @@ def cpu_baseline(budget_s: float):
                 break
     except OSError:
         pass
-    return {"value": round(flop / mean / 1e12, 4), "unit": "TFLOP/s", "cores": cores,
-            "kind": "port",
-            "sample": f"full 8192^3 FP32 mtm (C += A*B), mean of {iters} timed calls after "
-                      f"1 warm-up ({t_first:.2f} s)",
-            "median_s": float(np.median(times)), "mean_s": mean, "cpu_model": model,
-            "omp_places": os.environ.get("OMP_PLACES"),
-            "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
-            "blocking": blk.as_dict(), "machine": platform.machine()}
+    res = {"value": round(flop / mean / 1e12, 4), "unit": "TFLOP/s", "cores": cores,
+           "kind": "port",
+           "sample": f"full 8192^3 FP32 mtm (C += A*B), mean of {iters} timed calls after "
+                     f"1 warm-up ({t_first:.2f} s)",
+           "median_s": float(np.median(times)), "mean_s": mean, "cpu_model": model,
+           "omp_places": os.environ.get("OMP_PLACES"),
+           "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+           "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+           "blocking": blk.as_dict(), "machine": platform.machine()}
+    res.update(host_cache_keys(blk))
+    return res
 
 
 if __name__ == "__main__":

@@ -130,7 +130,8 @@ omb_status omb_fill_uniform(float* dst, int64_t n, uint64_t seed, uint64_t strea
 /* ---------------------------------------------------- FP64 error checks ----
  * Measurement kernels (not part of the FP32 path): an FP64 GEMM / GEMV / dot
  * of FP32 inputs and an error norm.  C64 is row-major M x N (ld = N).
- * c0 (nullable, strides of C) is added first, so C64 = c0 + A*B. */
+ * c0 (nullable, strides of C) is added to the FP64 product: C64 = c0 + A*B.
+ * Both validated against host FP64 (Higham's gamma_K bound). */
 omb_status omb_check_dgemm(int64_t M, int64_t N, int64_t K,
                            const float* A, int64_t rsA, int64_t csA,
                            const float* B, int64_t rsB, int64_t csB,
@@ -157,7 +158,10 @@ omb_status omb_calib_fmac(int64_t blocks, int64_t iters, float* sink, omb_stream
  * (y_j = alpha*A_j*x_j + beta*y_j, A_j row-major) executed by ONE
  * persistent kernel launch, captured in a hipGraph.  No counterpart in the
  * reference (it has no batched API); the per-instance semantics are those
- * of omb_sgemm / omb_sgemv. */
+ * of omb_sgemm / omb_sgemv, and each output is bitwise the direct call's,
+ * except where omb_sgemm splits K into coarse blocks (K > 512 with few
+ * output tiles, e.g. 2048^3 plain): grouped tiles keep the unsplit fold,
+ * which rounds differently within the same error bound. */
 typedef struct {
     int64_t M, N, K;
     const float* A; int64_t lda;

@@ -28,7 +28,9 @@ import re
 from typing import Optional
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libompblas.so")
+# OMB_LIBRARY names another build of the same library (e.g. one whose host
+# code is compiled with sanitizers); by default the in-tree build is loaded.
+LIB_PATH = os.environ.get("OMB_LIBRARY") or os.path.join(_HERE, "lib", "libompblas.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "ompblas.h")
 
 OK = 0

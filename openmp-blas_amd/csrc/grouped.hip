@@ -14,7 +14,11 @@
 // slices, product order and 256-k folds; scaled split-FP16: the same line
 // exponents, pieces and products; GEMV: the same lane order and butterfly),
 // so a grouped result is bitwise the one omb_sgemm / omb_sgemv gives for
-// that instance alone -- whatever else is in the batch.
+// that instance alone -- whatever else is in the batch -- with one
+// exception: for K > 512 and few output tiles omb_sgemm may split K into
+// coarse blocks of several accumulation blocks each, which rounds
+// differently; grouped tiles never split K, so such an instance keeps the
+// unsplit fold (same error bound, other bits).
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -28,6 +32,7 @@
 #include <vector>
 
 #include "gemv_row.hpp"
+#include "host_util.hpp"
 #include "split_common.hpp"
 
 namespace omb {
@@ -64,9 +69,6 @@ struct DGemv {
     float alpha, beta;
 };
 
-enum Cls : int { C32 = 0, C64 = 1, C128x64 = 2, CSPLIT = 3, CGEMV = 4, NCLS = 5 };
-const char* const kClsName[NCLS] = {"f32_32x32", "f32_64x64", "f32_128x64", "bf16x3_64x64",
-                                    "gemv_64rows"};
 constexpr int kTileM[NCLS] = {32, 64, 128, 64, 64};
 constexpr int kTileN[NCLS] = {32, 64, 64, 64, 1};
 
@@ -511,12 +513,7 @@ __global__ __launch_bounds__(NT, BIG ? 2 : 3) void grouped_kernel(PArgs p) {
 }
 
 // ---------------------------------------------------------- tuning table --
-struct Key {
-    int mode, bm, bn, bk;
-    bool operator<(const Key& o) const {
-        return std::tie(mode, bm, bn, bk) < std::tie(o.mode, o.bm, o.bn, o.bk);
-    }
-};
+using Key = TuneKey;
 std::mutex g_tune_mu;
 std::map<Key, int>& tune_table() {
     static auto* t = new std::map<Key, int>();
@@ -545,10 +542,14 @@ int default_class(int mode, const omb_gemm_desc& d) {
     return std::min(d.M, d.N) < 48 ? C32 : C64;
 }
 
+// The table's class for the bucket, if it computes this mode (the loader
+// accepts only such entries), else the default.
 int choose_class(int mode, const omb_gemm_desc& d) {
     std::lock_guard<std::mutex> lock(g_tune_mu);
     auto it = tune_table().find(key_of(mode, d));
-    return it != tune_table().end() ? it->second : default_class(mode, d);
+    return it != tune_table().end() && class_serves_mode(it->second, mode)
+               ? it->second
+               : default_class(mode, d);
 }
 
 }  // namespace
@@ -1005,24 +1006,10 @@ extern "C" omb_status omb_autotune_load(const char* table_path) {
         return OMB_EINVAL;
     }
     std::map<Key, int> loaded;
-    std::string line;
-    while (std::getline(f, line)) {
-        if (line.empty() || line[0] == '#') continue;
-        std::istringstream is(line);
-        Key k{};
-        std::string name;
-        if (!(is >> k.mode >> k.bm >> k.bn >> k.bk >> name)) {
-            set_error("omb_autotune_load: malformed line: " + line);
-            return OMB_EINVAL;
-        }
-        int cls = -1;
-        for (int c = 0; c < NCLS; ++c)
-            if (name == kClsName[c]) cls = c;
-        if (cls < 0 || cls == CGEMV) {
-            set_error("omb_autotune_load: unknown class " + name);
-            return OMB_EINVAL;
-        }
-        loaded[k] = cls;
+    std::string err;
+    if (!parse_tune_table(f, &loaded, &err)) {
+        set_error("omb_autotune_load: " + err);
+        return OMB_EINVAL;
     }
     std::lock_guard<std::mutex> lock(g_tune_mu);
     for (auto& kv : loaded) tune_table()[kv.first] = kv.second;

@@ -25,78 +25,53 @@
 #include <vector>
 
 #include "common.hpp"
+#include "host_util.hpp"
 
 namespace omb {
 namespace {
 
 using Clock = std::chrono::steady_clock;
 
-// First-fit allocator over one device buffer (256-byte granules).
+// One device buffer handed out first-fit in 256-byte granules.
 class Arena {
    public:
     ~Arena() {
         if (base_) (void)hipFree(base_);
     }
     omb_status init(size_t bytes) {
-        size_ = round_up((int64_t)std::max<size_t>(bytes, 256), 256);
-        if (hipMalloc(&base_, size_) != hipSuccess) {
+        fit_.reset(bytes);
+        if (hipMalloc(&base_, fit_.capacity()) != hipSuccess) {
             (void)hipGetLastError();
             base_ = nullptr;
-            set_error("serve: arena allocation of " + std::to_string(size_) + " bytes failed");
+            set_error("serve: arena allocation of " + std::to_string(fit_.capacity()) +
+                      " bytes failed");
             return OMB_ENOMEM;
         }
-        free_.clear();
-        free_.push_back({0, size_});
         return OMB_OK;
     }
-    size_t capacity() const { return size_; }
-    bool alloc(size_t bytes, size_t* off) {
-        bytes = round_up((int64_t)std::max<size_t>(bytes, 1), 256);
-        for (auto it = free_.begin(); it != free_.end(); ++it) {
-            if (it->size >= bytes) {
-                *off = it->off;
-                it->off += bytes;
-                it->size -= bytes;
-                if (it->size == 0) free_.erase(it);
-                return true;
-            }
-        }
-        return false;
-    }
-    void release(size_t off, size_t bytes) {
-        bytes = round_up((int64_t)std::max<size_t>(bytes, 1), 256);
-        auto it = free_.begin();
-        while (it != free_.end() && it->off < off) ++it;
-        it = free_.insert(it, {off, bytes});
-        auto next = std::next(it);
-        if (next != free_.end() && it->off + it->size == next->off) {
-            it->size += next->size;
-            free_.erase(next);
-        }
-        if (it != free_.begin()) {
-            auto prev = std::prev(it);
-            if (prev->off + prev->size == it->off) {
-                prev->size += it->size;
-                free_.erase(it);
-            }
-        }
-    }
+    size_t capacity() const { return fit_.capacity(); }
+    bool alloc(size_t bytes, size_t* off) { return fit_.alloc(bytes, off); }
+    void release(size_t off, size_t bytes) { fit_.release(off, bytes); }
     char* base() const { return static_cast<char*>(base_); }
 
    private:
-    struct Block {
-        size_t off, size;
-    };
     void* base_ = nullptr;
-    size_t size_ = 0;
-    std::list<Block> free_;
+    FirstFit fit_;
 };
+
+// Planes a split-BF16 request packs: the 3-product form reads only the
+// first two, so it packs and stages two, as omb_sgemm does.
+int split_planes(const omb_request& r) { return r.mode == OMB_MODE_SPLIT_BF16_P3 ? 2 : 3; }
+
+int64_t packed_elems(const omb_request& r, int64_t rows) {
+    return split_planes(r) == 2 ? split16_packed_elems(rows, r.k) : split_packed_elems(rows, r.k);
+}
 
 // Packed split-BF16 operands of a GEMM request, rounded to 256 bytes (the
 // split partial sums, when K is split, follow them).
 size_t packed_bytes(const omb_request& r) {
-    return (size_t)round_up(
-        (split_packed_elems(r.m, r.k) + split_packed_elems(r.n, r.k)) * (int64_t)sizeof(__bf16), 256);
+    return (size_t)round_up((packed_elems(r, r.m) + packed_elems(r, r.n)) * (int64_t)sizeof(__bf16),
+                            256);
 }
 
 size_t need_bytes(const omb_request& r) {
@@ -161,13 +136,15 @@ omb_status launch_request(const omb_request& r, char* ws, hipStream_t s) {
             if (r.mode == OMB_MODE_SPLIT_FP16)
                 return launch_sgemm_split16_ws(r.m, r.n, r.k, 1.f, r.a, r.k, 1, r.b, r.n, 1, 0.f,
                                                r.c, r.n, ws, s);
+            const int planes = split_planes(r);
             __bf16* ap = reinterpret_cast<__bf16*>(ws);
-            __bf16* bp = ap + split_packed_elems(r.m, r.k);
-            OMB_TRY(launch_split_pack(r.m, r.k, r.a, r.k, 1, ap, s));
-            OMB_TRY(launch_split_pack(r.n, r.k, r.b, 1, r.n, bp, s));  // B^T of row-major B
+            __bf16* bp = ap + packed_elems(r, r.m);
+            OMB_TRY(launch_split_pack(r.m, r.k, r.a, r.k, 1, ap, s, planes));
+            OMB_TRY(launch_split_pack(r.n, r.k, r.b, 1, r.n, bp, s, planes));  // B^T of row-major B
             return launch_sgemm_packed(r.m, r.n, r.k, 1.f, ap, bp, 0.f, r.c, r.n,
-                                       r.mode == OMB_MODE_SPLIT_BF16_P3 ? 3 : 6, s,
-                                       reinterpret_cast<float*>(ws + packed_bytes(r)));
+                                       planes == 2 ? 3 : 6, s,
+                                       reinterpret_cast<float*>(ws + packed_bytes(r)), nullptr,
+                                       nullptr, planes);
         }
         case OMB_REQ_GEMV:
             if (r.m == 0) return OMB_OK;
@@ -315,6 +292,11 @@ extern "C" omb_status omb_serve_run(omb_request* reqs, int32_t n, const omb_serv
             results[r].slot = si;
             if (deferred[r]) ++late;
             if (st != OMB_OK) {
+                (void)hipGetLastError();
+                // Kernels of this request may already be queued (a split pack
+                // before a failed launch): the block goes back to the arena
+                // only once the slot's stream has drained.
+                (void)hipStreamSynchronize(s.stream);
                 (void)hipGetLastError();
                 results[r].status = st;
                 results[r].done_s = ts;

@@ -9,6 +9,10 @@
  * this image (no Boost headers), so this restatement is the oracle.
  *
  * Parity pins (see tests/test_oracle.py):
+ *   - oracle_cache_probe    : the host's cache parameters as getconf(1) reports
+ *                             them, placed in the slots of
+ *                             include/cache_manager.hpp:107-150 (L3
+ *                             associativity in the L2 slot, L3's left at 1).
  *   - oracle_blocking       : doc tables (KB 384/256, MB 32/32, NB 4776/3582,
  *                             MR/NR 16x6 / 8x6) from doc/matrix_times_matrix.tex
  *                             :290-302,378-388,440-450,487-497 with the doc

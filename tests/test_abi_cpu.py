@@ -165,6 +165,10 @@ def test_serve_request_bytes(ompblas):
     f16 = ompblas.serve_request_bytes(R(ompblas.REQ_GEMM, 3, 300, 200, 260, 0.0, 16, 16, 16, 0))
     assert f16 % 256 == 0 and f16 >= 2 * (300 + 200) * 260 * 2 + (300 + 200) * 8
     assert f16 < split
+    # the 3-product BF16 form packs only the two planes it reads
+    p3 = ompblas.serve_request_bytes(R(ompblas.REQ_GEMM, 2, 300, 200, 260, 0.0, 16, 16, 16, 0))
+    two_planes = 2 * (384 + 256) * 272 * 2          # rows padded to 128, k to 16
+    assert p3 == (two_planes + 255) // 256 * 256 and p3 < split
     assert ompblas.serve_request_bytes(R(ompblas.REQ_GEMV, 0, 30, 20, 0, 0.0, 16, 16, 16, 0)) == 0
     assert ompblas.serve_request_bytes(R(ompblas.REQ_DOT, 0, 0, 5000, 0, 0.0, 16, 16, 16, 0)) > 0
     with pytest.raises(ompblas.OmbError):
@@ -198,7 +202,7 @@ def test_cpp_dropin_program_is_built_against_the_library(ompblas):
     linked to the product library (not to the oracle)."""
     import os
     import subprocess
-    path = os.path.join(os.path.dirname(ompblas.LIB_PATH), "test_dropin")
+    path = os.path.join(os.path.dirname(os.path.abspath(ompblas.__file__)), "lib", "test_dropin")
     assert os.path.exists(path)
     deps = subprocess.run(["ldd", path], capture_output=True, text=True).stdout
     assert "libompblas.so" in deps and "oracle" not in deps
@@ -231,3 +235,16 @@ def test_outer_and_transpose_mirror_shape_errors(ompblas):
         ompblas.transpose(torch.zeros(3, 4), torch.zeros(3, 4))
     with pytest.raises(RuntimeError, match="square"):
         ompblas.transpose(torch.zeros(3, 4))
+
+
+@pytest.mark.parametrize("line", ["0 4 5 6 bf16x3_64x64", "1 4 5 6 f32_64x64",
+                                  "3 4 5 6 f32_32x32", "9 4 5 6 f32_32x32",
+                                  "0 4 5 6 f32_32x32 trailing"])
+def test_autotune_load_refuses_class_of_another_mode(ompblas, tmp_path, line):
+    """A plain-FP32 line naming the split tile, or a split line naming a plain
+    tile, would make a plan compute another mode's arithmetic: refused with
+    OMB_EINVAL before the table changes (no device call)."""
+    path = tmp_path / "t.txt"
+    path.write_text("# mode log2M log2N log2K class\n" + line + "\n")
+    assert ompblas.lib().omb_autotune_load(str(path).encode()) == ompblas.EINVAL
+    assert "omb_autotune_load" in ompblas.last_error()

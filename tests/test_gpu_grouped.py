@@ -225,3 +225,91 @@ def test_grouped_fp16_exponents_see_every_element(dev, ompblas):
     torch.cuda.synchronize()
     assert torch.equal(b.gc, want_c)
     plan.close()
+
+
+def c4_shapes(ompblas):
+    """The grouped configuration's batch exactly as the benchmark draws it:
+    seed 3, 4096 GEMMs with M, N, K log-uniform on [16, 512], then 4096
+    GEMVs with rows and cols log-uniform on [64, 4096]."""
+    rng = np.random.default_rng(3)
+    g = np.stack([ompblas.log_uniform_ints(rng, 16, 512, 4096) for _ in range(3)], 1)
+    v = np.stack([ompblas.log_uniform_ints(rng, 64, 4096, 4096) for _ in range(2)], 1)
+    return [tuple(map(int, s)) for s in g], [tuple(map(int, s)) for s in v]
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_grouped_c4_full_batch(dev, ompblas, mode):
+    """The full 4096 + 4096 batch in plain FP32 and split-BF16: the captured
+    plan (replayed twice) is bitwise equal to one call per instance; then
+    with integer operands in [-4, 4] every GEMM and GEMV output of the plan
+    equals the exact integer product (partial sums stay below 2^24)."""
+    gs, vs = c4_shapes(ompblas)
+    b = Batch(dev, ompblas, gs, vs, seed=3)
+    want_c, want_y = b.eager(ompblas, mode)
+    plan = ompblas.GroupPlan(b.gemm_descs(), b.gemv_descs(), mode)
+    print(plan.describe())
+    for replay in range(2):
+        b.reset()
+        torch.cuda.synchronize()
+        plan.launch()
+        torch.cuda.synchronize()
+        assert torch.equal(b.gc, want_c), (mode, replay)
+        assert torch.equal(b.vy, want_y), (mode, replay)
+    del want_c, want_y
+    # exact answers: U[-1,1) * 4 rounded in place gives integers in [-4, 4]
+    for t in (b.ga, b.gb, b.va, b.vx):
+        t.mul_(4.0).round_()
+    b.c0.zero_()
+    b.y0.zero_()
+    b.reset()
+    torch.cuda.synchronize()
+    plan.launch()
+    torch.cuda.synchronize()
+    for i, (m, n, k) in enumerate(gs):
+        a = b.ga[b.oa[i]:b.oa[i] + m * k].view(m, k).double()
+        bb = b.gb[b.ob[i]:b.ob[i] + k * n].view(k, n).double()
+        c = b.gc[b.oc[i]:b.oc[i] + m * n].view(m, n).double()
+        assert torch.equal(c, a @ bb), (mode, i, m, n, k)
+    for i, (m, n) in enumerate(vs):
+        a = b.va[b.ova[i]:b.ova[i] + m * n].view(m, n).double()
+        x = b.vx[b.ovx[i]:b.ovx[i] + n].double()
+        y = b.vy[b.ovy[i]:b.ovy[i] + m].double()
+        assert torch.equal(y, a @ x), (mode, i, m, n)
+    plan.close()
+
+
+def test_grouped_instance_where_the_direct_call_splits_k_coarsely(dev, ompblas):
+    """2048^3 plain FP32: omb_sgemm may split K into coarse blocks of several
+    accumulation blocks each, which rounds differently from the unsplit fold
+    that grouped tiles keep, so bitwise equality is not promised for such an
+    instance.  What holds: the grouped result is the same on every replay,
+    its error against FP64 is within 2x the oracle's on a 64-row slab, and
+    the small instances beside it stay bitwise equal to their direct calls."""
+    gs = [(2048, 2048, 2048), (100, 70, 300), (33, 512, 129)]
+    b = Batch(dev, ompblas, gs, [], seed=31)
+    want_c, _ = b.eager(ompblas, 0)
+    plan = ompblas.GroupPlan(b.gemm_descs(), [], 0)
+    outs = []
+    for _ in range(2):
+        b.reset()
+        plan.launch()
+        torch.cuda.synchronize()
+        outs.append(b.gc.clone())
+    assert torch.equal(outs[0], outs[1])
+    for i, (m, n, k) in enumerate(gs[1:], start=1):
+        sl = slice(b.oc[i], b.oc[i] + m * n)
+        assert torch.equal(outs[0][sl], want_c[sl]), i
+    n = 2048
+    a = b.ga[:n * n].view(n, n).cpu().numpy()
+    bb = b.gb[:n * n].view(n, n).cpu().numpy()
+    c = outs[0][:n * n].view(n, n).cpu().numpy()
+    rows = slice(1000, 1064)
+    truth = a[rows].astype(np.float64) @ bb.astype(np.float64)
+    c_ref = np.zeros((64, n), np.float32)
+    helpers.oracle_mtm(c_ref, np.ascontiguousarray(a[rows]), bb)
+    ref = helpers.rel_errors(c_ref, truth)
+    got = helpers.rel_errors(c[rows], truth)
+    direct = helpers.rel_errors(want_c[:n * n].view(n, n)[rows].cpu().numpy(), truth)
+    print("grouped", got, "direct", direct, "oracle", ref)
+    assert got[0] <= 2 * ref[0] and got[1] <= 2 * ref[1]
+    plan.close()

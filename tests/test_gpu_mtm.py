@@ -349,6 +349,42 @@ def test_mtm_8192_slab_error_vs_reference(dev, ompblas, layout):
         assert got[0] <= 2 * ref[0] and got[1] <= 2 * ref[1], (mode, got, ref)
 
 
+@pytest.mark.parametrize("dist", ["uniform", "pow2"])
+def test_mtm_split_config_8192_both_distributions(dev, ompblas, dist):
+    """The split-precision configuration: 8192^3 row-major from seed 2,
+    U[-1,1) and sign * 2^u with u ~ U[-8, 8].  On a 64-row slab every
+    accurate mode is within 2x the oracle's error on the same rows; on the
+    whole matrix (FP64 check product) the split modes are within 2x plain
+    FP32's error."""
+    n = 8192
+    if dist == "uniform":
+        a = helpers.rng_uniform(n * n, 2, 0).reshape(n, n)
+        b = helpers.rng_uniform(n * n, 2, 1).reshape(n, n)
+    else:
+        a = helpers.rng_pow2(n * n, 2, 0).reshape(n, n)
+        b = helpers.rng_pow2(n * n, 2, 1).reshape(n, n)
+    rows = slice(4096, 4160)
+    truth = a[rows].astype(np.float64) @ b.astype(np.float64)
+    c_ref = np.zeros((64, n), np.float32)
+    oracle_mtm(c_ref, np.ascontiguousarray(a[rows]), b)
+    ref = rel_errors(c_ref, truth)
+    ta, tb = to_dev(a, dev), to_dev(b, dev)
+    del a, b
+    c64 = ompblas.check_dgemm(ta, tb)
+    full = {}
+    for mode in ACCURATE:
+        tc = torch.zeros((n, n), device=dev)
+        run_mtm(ompblas, tc, ta, tb, mode)
+        got = rel_errors(tc[rows].cpu().numpy(), truth)
+        full[mode] = ompblas.rel_errors(tc, c64)
+        print(dist, mode, "slab gpu", got, "oracle", ref, "full", full[mode])
+        assert got[0] <= 2 * ref[0] and got[1] <= 2 * ref[1], (dist, mode, got, ref)
+        del tc
+    for mode in (1, 3):
+        assert full[mode][0] <= 2 * full[0][0] and full[mode][1] <= 2 * full[0][1], (
+            dist, mode, full)
+
+
 @pytest.mark.parametrize("nprod", [6, 3])
 @pytest.mark.parametrize("shape", [(256, 384, 512), (130, 77, 1000), (1, 129, 17)])
 def test_packed_split_path_matches_one_call(dev, ompblas, shape, nprod):

@@ -70,7 +70,7 @@ def small_trace(ompblas, count, rate, seed):
                                  dot=(1 << 10, 1 << 20))
 
 
-@pytest.mark.parametrize("mode", [1, 0, 3])
+@pytest.mark.parametrize("mode", [1, 0, 2, 3])
 def test_mix_equals_solo_bitwise(dev, ompblas, mode):
     """High arrival rate, so requests overlap on the device; each output is
     bitwise equal to its solo run."""
@@ -129,7 +129,7 @@ def test_small_budget_makes_requests_wait(dev, ompblas):
         assert torch.equal(mixed[i], tr.solo(i)), i
 
 
-@pytest.mark.parametrize("mode", [0, 1, 3])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_split_k_requests_equal_solo(dev, ompblas, mode):
     """GEMMs with few output tiles and long K take the split-K path, whose
     partial sums live in the request's workspace from the arena; overlapped
@@ -192,3 +192,104 @@ def test_empty_and_degenerate_requests(dev, ompblas):
     assert torch.isnan(out[3]).all()      # M == 0: nothing written
     res, st = ompblas.serve_run([], slots=2)
     assert res == [] and st["p99_s"] == 0.0
+
+
+def integer_operands(tr, lo, hi):
+    """Replace every request's operands by integers: U[-1,1) scaled to
+    [lo, hi] and rounded in place (GEMM and GEMV entries), or rounded to
+    {-1, 0, 1} (dot entries, whose length reaches 2^24)."""
+    p = tr.ompblas
+    for kind, m, n, k, t, a, b, c in tr.items:
+        for v in (a, b):
+            if kind == p.REQ_DOT:
+                v.round_()
+            else:
+                v.mul_((hi - lo) / 2.0).add_((hi + lo) / 2.0).round_()
+
+
+@pytest.mark.parametrize("mode", [1, 0, 3])
+def test_full_range_requests_give_exact_integer_answers(dev, ompblas, mode):
+    """Requests drawn at the serving configuration's full ranges (GEMM M, N,
+    K up to 4096, GEMV up to 16384 x 16384, dot up to 2^24) with integer
+    operands: every output of the loop equals the exact integer result --
+    |entries| <= 4 keeps every GEMM / GEMV partial sum below 2^24, and dot
+    entries in {-1, 0, 1} keep its sums at most 2^24, so FP32 holds them
+    exactly and the split modes carry small integers without rounding."""
+    trace = ompblas.serving_trace(120, 4.0e4, seed=40 + mode)
+    # the largest of each kind among the trace is included
+    trace += [(ompblas.REQ_GEMM, 4096, 4096, 4096, trace[-1][4]),
+              (ompblas.REQ_GEMV, 16384, 16384, 0, trace[-1][4]),
+              (ompblas.REQ_DOT, 0, 1 << 24, 0, trace[-1][4])]
+    tr = Trace(dev, ompblas, trace, seed=40 + mode, mode=mode)
+    integer_operands(tr, -4, 4)
+    torch.cuda.synchronize()
+    res, st = ompblas.serve_run(tr.requests(), slots=8)
+    torch.cuda.synchronize()
+    assert all(r["status"] == 0 for r in res)
+    assert st["max_in_flight"] > 1, st
+    for i, (kind, m, n, k, t, a, b, c) in enumerate(tr.items):
+        if kind == ompblas.REQ_GEMM:
+            want = a.view(m, k).double() @ b.view(k, n).double()
+            got = c.view(m, n).double()
+        elif kind == ompblas.REQ_GEMV:
+            want = a.view(m, n).double() @ b.double()
+            got = c.double()
+        else:
+            want = torch.dot(a.double(), b.double()).view(1)
+            got = c.double()
+        assert torch.equal(got, want), (i, trace[i])
+
+
+def test_serving_config_trace_matches_solo_and_oracle(dev, ompblas):
+    """The serving configuration's trace as the benchmark draws it (seed 4,
+    default ranges: GEMM 32-4096, GEMV 256-16384, dot 2^10-2^24) at 300
+    requests and the stated 32,000 requests/s, split-BF16 GEMMs: every output
+    is bitwise its solo run; the three largest GEMMs, GEMVs and dots are
+    within 2x the oracle's error against an FP64 product on the host (the
+    reference algorithm on the same inputs: include/mtm.hpp:116-206,
+    include/mtv.hpp:72-98, include/dot.hpp:143-174)."""
+    trace = ompblas.serving_trace(300, 3.2e4, seed=4)
+    tr = Trace(dev, ompblas, trace, seed=4, mode=ompblas.SPLIT_BF16)
+    res, st = ompblas.serve_run(tr.requests(), slots=8)
+    torch.cuda.synchronize()
+    assert all(r["status"] == 0 for r in res)
+    print({k: st[k] for k in ("p50_s", "p99_s", "max_in_flight", "wall_s")})
+    mixed = tr.outputs()
+    for i in range(len(trace)):
+        assert torch.equal(mixed[i], tr.solo(i)), (i, trace[i])
+
+    def largest(kind, size):
+        idx = [i for i, t in enumerate(trace) if t[0] == kind]
+        return sorted(idx, key=lambda i: size(trace[i]), reverse=True)[:3]
+
+    for i in largest(ompblas.REQ_GEMM, lambda t: t[1] * t[2] * t[3]):
+        _, m, n, k, _, a, b, c = tr.items[i]
+        ah, bh = a.view(m, k).cpu().numpy(), b.view(k, n).cpu().numpy()
+        truth = ah.astype(np.float64) @ bh.astype(np.float64)
+        c_ref = np.zeros((m, n), np.float32)
+        helpers.oracle_mtm(c_ref, ah, bh)
+        ref = helpers.rel_errors(c_ref, truth)
+        got = helpers.rel_errors(mixed[i].view(m, n).cpu().numpy(), truth)
+        print("gemm", (m, n, k), "gpu", got, "oracle", ref)
+        assert got[0] <= 2 * ref[0] and got[1] <= 2 * ref[1], (i, trace[i], got, ref)
+    for i in largest(ompblas.REQ_GEMV, lambda t: t[1] * t[2]):
+        _, m, n, k, _, a, b, c = tr.items[i]
+        ah, xh = a.view(m, n).cpu().numpy(), b.cpu().numpy()
+        truth = ah.astype(np.float64) @ xh.astype(np.float64)
+        y_ref = np.zeros(m, np.float32)
+        helpers.oracle_mtv(y_ref, ah, xh)
+        ref = helpers.rel_errors(y_ref, truth)
+        got = helpers.rel_errors(mixed[i].cpu().numpy(), truth)
+        print("gemv", (m, n), "gpu", got, "oracle", ref)
+        assert got[0] <= 2 * ref[0] and got[1] <= 2 * ref[1], (i, trace[i], got, ref)
+    for i in largest(ompblas.REQ_DOT, lambda t: t[2]):
+        _, m, n, k, _, a, b, c = tr.items[i]
+        xh, yh = a.cpu().numpy(), b.cpu().numpy()
+        truth = float(np.dot(xh.astype(np.float64), yh.astype(np.float64)))
+        absdot = float(np.dot(np.abs(xh).astype(np.float64), np.abs(yh).astype(np.float64)))
+        ref = abs(helpers.oracle_dot(xh, yh) - truth)
+        got = abs(float(mixed[i].item()) - truth)
+        print("dot", n, "gpu", got, "oracle", ref)
+        # one scalar's error can be small by chance: the larger of 2x the
+        # oracle's and a fixed fraction of the summation bound n u sum|xy|
+        assert got <= max(2 * ref, 1e-3 * n * 2.0 ** -24 * absdot), (i, got, ref)

@@ -25,6 +25,52 @@ def test_ct_sqrt_quirk():
     assert o.oracle_ct_sqrt(36.0) == 6.375
 
 
+def _getconf(name: str) -> int:
+    """One cache parameter from getconf(1) (0 when the system does not know it)."""
+    import subprocess
+    out = subprocess.run(["getconf", name], capture_output=True, text=True)
+    try:
+        return int(out.stdout.strip())
+    except ValueError:
+        return 0
+
+
+def _int8(v: int) -> int:
+    """The reference stores associativity as int8_t (include/cache_manager.hpp:30-31)."""
+    return ((v + 128) % 256) - 128
+
+
+def test_cache_probe_matches_this_host():
+    """oracle_cache_probe against the host's cache parameters as getconf(1)
+    reports them, with the reference's Linux slot assignment
+    (include/cache_manager.hpp:107-150): sizes and line sizes of L1d/L2/L3 in
+    their own slots, the L1 associativity in slot 0, the L3 associativity in
+    the L2 slot (:133-135, overwriting the L2 value when L3's is known), and
+    slot 2 left at the default associativity 1.  Unknown (<= 0) values keep
+    the defaults: size and line 0, associativity 1."""
+    import ctypes
+    raw = {lv: (_getconf(f"LEVEL{lv}_CACHE_SIZE" if lv > 1 else "LEVEL1_DCACHE_SIZE"),
+                _getconf(f"LEVEL{lv}_CACHE_ASSOC" if lv > 1 else "LEVEL1_DCACHE_ASSOC"),
+                _getconf(f"LEVEL{lv}_CACHE_LINESIZE" if lv > 1 else "LEVEL1_DCACHE_LINESIZE"))
+           for lv in (1, 2, 3)}
+    g = helpers.CacheGeom()
+    helpers.oracle().oracle_cache_probe(ctypes.byref(g))
+    got = [(g.l[i].size, g.l[i].assoc, g.l[i].line) for i in range(3)]
+    pos = lambda v, d: v if v > 0 else d
+    want_assoc = [_int8(pos(raw[1][1], 1)),
+                  _int8(raw[3][1]) if raw[3][1] > 0 else _int8(pos(raw[2][1], 1)),
+                  1]
+    want = [(pos(raw[lv][0], 0), want_assoc[lv - 1], pos(raw[lv][2], 0)) for lv in (1, 2, 3)]
+    assert got == want, (got, raw)
+    # On a host whose L2 and L3 associativities differ, the slot bug is
+    # visible: the L2 slot holds L3's value, not L2's.
+    if raw[2][1] > 0 and raw[3][1] > 0 and raw[2][1] != raw[3][1]:
+        assert got[1][1] == raw[3][1] != raw[2][1]
+    # The blocking the oracle uses on this host is the one its probe gives.
+    if all(v[0] > 0 and v[2] > 0 for v in raw.values()):
+        assert helpers.host_blocking().as_dict() == blocking(got, 4, 0).as_dict()
+
+
 def test_blocking_linux_l3_assoc_slot():
     """include/cache_manager.hpp:133-135 writes the L3 associativity into the L2
     slot; with this container's geometry that gives MC = 464 and NC = 23520."""
