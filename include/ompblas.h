@@ -103,6 +103,18 @@ omb_status omb_sgemv(int64_t M, int64_t N, float alpha,
 omb_status omb_sdot(int64_t n, const float* x, const float* y, float* result,
                      omb_stream_t stream);
 
+/* omb_sdot without allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * kernels and bits as omb_sdot; the per-workgroup partial sums (at most
+ * 1024 doubles, 8 KiB, for any n) live in a static device array of the
+ * library instead of the stream's workspace.  Every call on a device shares
+ * that array: calls that overlap on one device (other streams, other host
+ * threads) race and must be serialised by the caller.  Its address is
+ * looked up once per device, in omb_init or on the first call there. */
+omb_status omb_sdot_noalloc(int64_t n, const float* x, const float* y, float* result,
+                            omb_stream_t stream);
+
 /* --------------------------------------------------------- outer product ----
  * C += alpha * x * y^T, C M x N in either storage order (rsC, csC); every
  * element is fma(alpha * x[i], y[j], C[i][j]).  Replaces

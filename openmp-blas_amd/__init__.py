@@ -115,6 +115,7 @@ _SIGS = {
                                 _VP]),
     "omb_sgemv": (_I32, [_I64, _I64, _F32, _VP, _I64, _I64, _VP, _F32, _VP, _VP]),
     "omb_sdot": (_I32, [_I64, _VP, _VP, _VP, _VP]),
+    "omb_sdot_noalloc": (_I32, [_I64, _VP, _VP, _VP, _VP]),
     "omb_sger": (_I32, [_I64, _I64, _F32, _VP, _VP, _VP, _I64, _I64, _VP]),
     "omb_stranspose": (_I32, [_I64, _I64, _VP, _I64, _I64, _VP, _I64, _I64, _VP]),
     "omb_stranspose_inplace": (_I32, [_I64, _VP, _I64, _VP]),

@@ -134,7 +134,7 @@ extern "C" omb_status omb_init(void) {
         set_error("omb_init: device is " + arch + "; this library is built for gfx950 only");
         return OMB_EUNSUPPORTED;
     }
-    return OMB_OK;
+    return sdot_scratch_setup(nullptr);  // omb_sdot_noalloc's scratch on this device
 }
 
 extern "C" omb_status omb_sgemm(int64_t M, int64_t N, int64_t K, float alpha, const float* A,
@@ -272,6 +272,16 @@ extern "C" omb_status omb_sdot(int64_t n, const float* x, const float* y, float*
         return OMB_EINVAL;
     }
     return launch_sdot(n, x, y, result, (hipStream_t)stream);
+}
+
+extern "C" omb_status omb_sdot_noalloc(int64_t n, const float* x, const float* y, float* result,
+                                       omb_stream_t stream) {
+    clear_error();
+    if (n < 0 || !result || (n > 0 && (!x || !y))) {
+        set_error("omb_sdot_noalloc: invalid argument");
+        return OMB_EINVAL;
+    }
+    return launch_sdot_noalloc(n, x, y, result, (hipStream_t)stream);
 }
 
 extern "C" omb_status omb_fill_uniform(float* dst, int64_t n, uint64_t seed, uint64_t stream_id,

@@ -137,6 +137,12 @@ omb_status launch_sdot(int64_t n, const float* x, const float* y, float* result,
 int64_t sdot_blocks(int64_t n);
 omb_status launch_sdot_ws(int64_t n, const float* x, const float* y, float* result,
                           double* part, hipStream_t s);
+// SDOT with its partials in the library's static device scratch (no
+// allocation); sdot_scratch_setup finds that scratch's address on the
+// current device (once per device) and optionally returns it.
+omb_status sdot_scratch_setup(double** out);
+omb_status launch_sdot_noalloc(int64_t n, const float* x, const float* y, float* result,
+                               hipStream_t s);
 
 omb_status launch_fill_uniform(float* dst, int64_t n, uint64_t seed, uint64_t stream_id,
                                int64_t offset, hipStream_t s);

@@ -13,6 +13,7 @@
 // not depend on alignment, so the result is bitwise reproducible for any
 // placement of the operands.
 #include <algorithm>
+#include <mutex>
 
 #include "common.hpp"
 
@@ -23,6 +24,19 @@ constexpr int NT = 256;
 constexpr int UNROLL = 4;
 constexpr int kMaxBlocks = 1024;
 constexpr int64_t kF64Combine = int64_t(1) << 26;
+
+}  // namespace
+}  // namespace omb
+
+// Partial sums of omb_sdot_noalloc: one double per workgroup, at most
+// kMaxBlocks (8 KiB), in device memory the library owns for the life of the
+// process (each device holds its own copy).  Host code never passes this
+// name to a kernel: it stands for the host-side shadow, not the device
+// address, which set-up obtains with hipGetSymbolAddress per device.
+static __device__ __align__(256) double scratch[omb::kMaxBlocks];
+
+namespace omb {
+namespace {
 
 __global__ __launch_bounds__(NT) void sdot_kernel(int64_t n, const float* __restrict__ x,
                                                   const float* __restrict__ y, float* result,
@@ -116,6 +130,49 @@ omb_status launch_sdot_ws(int64_t n, const float* x, const float* y, float* resu
                            result);
     OMB_HIP_TRY(hipGetLastError());
     return OMB_OK;
+}
+
+// Device address of `scratch` on each device, filled once per device with
+// that device current (omb_init, or the first omb_sdot_noalloc there).
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+}  // namespace
+
+omb_status sdot_scratch_setup(double** out) {
+    int dev = 0;
+    OMB_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("sdot scratch: device ordinal " + std::to_string(dev) + " out of range");
+        return OMB_EUNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        OMB_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) {
+            set_error("sdot scratch: hipGetSymbolAddress returned no address");
+            return OMB_EDEVICE;
+        }
+        g_scratch[dev] = static_cast<double*>(p);
+    }
+    if (out) *out = g_scratch[dev];
+    return OMB_OK;
+}
+
+omb_status launch_sdot_noalloc(int64_t n, const float* x, const float* y, float* result,
+                               hipStream_t s) {
+    double* part = nullptr;
+    OMB_TRY(sdot_scratch_setup(&part));
+    const int64_t blocks = n > 0 ? sdot_blocks(n) : 0;
+    static_assert(sizeof(scratch) == kMaxBlocks * sizeof(double), "one double per workgroup");
+    if (blocks > kMaxBlocks) {
+        set_error("omb_sdot_noalloc: " + std::to_string(blocks) +
+                  " partial sums exceed the static scratch of " + std::to_string(kMaxBlocks));
+        return OMB_EUNSUPPORTED;
+    }
+    return launch_sdot_ws(n, x, y, result, part, s);
 }
 
 omb_status launch_sdot(int64_t n, const float* x, const float* y, float* result, hipStream_t s) {
