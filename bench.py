@@ -223,6 +223,45 @@ def main():
         "ms": t_z * 1e3, "mfma_tflops": 6 * flop / t_z / 1e12,
         "frac_of_bf16_peak": 6 * flop / t_z / PEAK_MFMA_BF16,
         "random_over_zero_time": t_k6 / t_z}
+    # What any BF16 MFMA kernel can sustain here: MFMAs on register operands
+    # only (no memory traffic), random against zero data, at the GEMM's
+    # occupancy (2 workgroups of 256 threads per CU) and for the same MFMA
+    # work (6 x 2 x 8192^3); and the vendor BF16 GEMM (torch.matmul on
+    # hipBLASLt) at 8192^3 on random and zero BF16 data, as a reference.
+    cus = torch.cuda.get_device_properties(dev).multi_processor_count
+    with torch.cuda.stream(stream):
+        sink = torch.zeros(1, device=dev)
+    mblocks = 2 * cus
+    miters = max(1, int(round(6 * flop / (mblocks * 4 * 64 * 32768.0))))
+    mwork = mblocks * 4 * miters * 64 * 32768.0
+    mceil = {}
+    for rnd in (1, 0):
+        fn = lambda: lib.omb_calib_mfma(mblocks, miters, rnd, sink.data_ptr(), sp)
+        pkg.check(fn())
+        mceil["random" if rnd else "zero"] = mwork / timer.run(fn, 3) / 1e12
+    kernels["mfma_ceiling (register operands, no memory traffic)"] = {
+        "random_tflops": mceil["random"], "zero_tflops": mceil["zero"],
+        "random_over_zero_time": mceil["zero"] / mceil["random"],
+        "work_flop": mwork, "kernel_frac_of_random_ceiling":
+            6 * flop / t_k6 / 1e12 / mceil["random"]}
+    vendor = {}
+    with torch.cuda.stream(stream):
+        Vb = torch.empty((n, n), dtype=torch.bfloat16, device=dev)
+        Wb = torch.empty((n, n), dtype=torch.bfloat16, device=dev)
+        Ob = torch.empty((n, n), dtype=torch.bfloat16, device=dev)
+        for name, fill in (("random", True), ("zero", False)):
+            if fill:
+                Vb.copy_(A)
+                Wb.copy_(B)
+            else:
+                Vb.zero_()
+                Wb.zero_()
+            torch.matmul(Vb, Wb, out=Ob)
+            vendor[name + "_tflops"] = flop / timer.run(lambda: torch.matmul(Vb, Wb, out=Ob),
+                                                        5) / 1e12
+    del Vb, Wb, Ob
+    kernels["vendor_bf16_gemm (torch.matmul 8192^3, hipBLASLt)"] = vendor
+
     kernels["split_pack_kernel (8192x8192)"] = {
         "ms": t_pack * 1e3, "gbps": (4 + 6) * n * n / t_pack / 1e9,
         "frac_of_hbm_peak": (4 + 6) * n * n / t_pack / PEAK_HBM}
@@ -432,7 +471,13 @@ def main():
                      "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                                      "scaled by a 256 MiB copy)",
                      "algorithmic_bytes": 2 * 3 * n * n * 2 + 4 * n * n,
-                     "work_per_launch": "6 products x 2*8192^3 = 6.597e12 MFMA flop"},
+                     "work_per_launch": "6 products x 2*8192^3 = 6.597e12 MFMA flop",
+                     "sustained_mfma_ceiling_random_operands": round(
+                         kernels["mfma_ceiling (register operands, no memory traffic)"]
+                         ["random_tflops"], 2),
+                     "frac_of_sustained_mfma_ceiling": round(
+                         kernels["mfma_ceiling (register operands, no memory traffic)"]
+                         ["kernel_frac_of_random_ceiling"], 4)},
         "cpu_baseline": cpu,
         "cpu_reference_path": cpu_ref,
         "config1_1024": c1,

@@ -163,6 +163,15 @@ omb_status omb_check_errnorm(int64_t M, int64_t N, const float* C, int64_t rsC, 
  * in the instruction form of the plain SGEMM kernel.  Timing it gives the
  * FP32 FMA rate the CUs sustain at that occupancy.  sink: 4 device bytes. */
 omb_status omb_calib_fmac(int64_t blocks, int64_t iters, float* sink, omb_stream_t stream);
+/* MFMA throughput ceiling (measurement only, no counterpart in the
+ * reference): `blocks` workgroups of 256 threads each run `iters` rounds of
+ * 64 v_mfma_f32_32x32x16_bf16 per wavefront (blocks * 4 * iters * 64 * 32768
+ * FLOP) on register operands -- random BF16 values (random != 0), each MFMA
+ * pairing other fragments, or all zero -- with no memory traffic in the
+ * loop.  Timing both gives the BF16 MFMA rate this GPU sustains with and
+ * without the data-dependent power of random operands. */
+omb_status omb_calib_mfma(int64_t blocks, int64_t iters, int32_t random, float* sink,
+                          omb_stream_t stream);
 
 /* ----------------------------------------------------- grouped launch ----
  * Thousands of independent GEMMs (C_i = alpha*A_i*B_i + beta*C_i, every

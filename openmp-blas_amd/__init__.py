@@ -126,6 +126,7 @@ _SIGS = {
     "omb_check_ddot": (_I32, [_I64, _VP, _VP, _VP, _VP]),
     "omb_check_errnorm": (_I32, [_I64, _I64, _VP, _I64, _I64, _VP, _VP, _VP]),
     "omb_calib_fmac": (_I32, [_I64, _I64, _VP, _VP]),
+    "omb_calib_mfma": (_I32, [_I64, _I64, _I32, _VP, _VP]),
     "omb_group_plan_create": (_I32, [ctypes.POINTER(GemmDesc), _I32, ctypes.POINTER(GemvDesc),
                                      _I32, _I32, _VP, ctypes.POINTER(_VP)]),
     "omb_group_plan_launch": (_I32, [_VP, _VP]),

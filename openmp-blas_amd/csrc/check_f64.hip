@@ -178,6 +178,49 @@ __global__ __launch_bounds__(NT) void fmac_ceiling_kernel(int64_t iters, float* 
     if (s == 0.125f) sink[0] = s;
 }
 
+// MFMA throughput ceiling: nothing but v_mfma_f32_32x32x16_bf16, 8
+// independent accumulators per wavefront, operands already in registers (no
+// memory traffic in the loop).  With random operands every MFMA multiplies
+// other data (8 A and 8 B fragments, 64 pairings per round), as the GEMM's
+// MFMAs do; with zero operands nothing toggles in the datapath.  Their rates
+// bound what any BF16 MFMA kernel sustains on this GPU with random data and
+// without that data-dependent power.  The single store is taken only when the
+// sum is exactly 0.125 (unreachable) and goes to sink[0].
+typedef __bf16 calib_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float calib_f32x16 __attribute__((ext_vector_type(16)));
+
+// `scale` (1 or 0) multiplies the random values: a runtime factor, so the
+// compiler cannot fold the zero-operand MFMAs away.
+__global__ __launch_bounds__(NT) void mfma_ceiling_kernel(int64_t iters, float scale, float* sink) {
+    calib_bf16x8 a[8], b[8];
+    const uint64_t key = rng_key(0x5eedULL, blockIdx.x);
+#pragma unroll
+    for (int v = 0; v < 8; ++v)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const uint64_t i = ((uint64_t)threadIdx.x * 8 + v) * 16 + e;
+            a[v][e] = (__bf16)(scale * rng_uniform(key, i));
+            b[v][e] = (__bf16)(scale * rng_uniform(key, i + 8));
+        }
+    calib_f32x16 acc[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[c] = calib_f32x16{};
+    for (int64_t it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                acc[(i + j) & 7] =
+                    __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[(i + j) & 7], 0, 0, 0);
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) s += acc[c][r];
+    if (s == 0.125f) sink[0] = s;
+}
+
 }  // namespace
 
 omb_status launch_fill_uniform(float* dst, int64_t n, uint64_t seed, uint64_t stream_id,
@@ -260,6 +303,19 @@ extern "C" omb_status omb_check_errnorm(int64_t M, int64_t N, const float* C, in
     hipLaunchKernelGGL(errnorm_partial_kernel, dim3(blocks), dim3(NT), 0, s, M, N, C, rsC, csC,
                        C64, part);
     hipLaunchKernelGGL(combine_kernel, dim3(1), dim3(1), 0, s, part, (int64_t)blocks, 4, out);
+    OMB_HIP_TRY(hipGetLastError());
+    return OMB_OK;
+}
+
+extern "C" omb_status omb_calib_mfma(int64_t blocks, int64_t iters, int32_t random, float* sink,
+                                     omb_stream_t stream) {
+    clear_error();
+    if (blocks < 1 || blocks > 0x7fffffff || iters < 1 || !sink) {
+        set_error("omb_calib_mfma: invalid argument");
+        return OMB_EINVAL;
+    }
+    hipLaunchKernelGGL(mfma_ceiling_kernel, dim3((unsigned)blocks), dim3(NT), 0,
+                       (hipStream_t)stream, iters, random ? 1.f : 0.f, sink);
     OMB_HIP_TRY(hipGetLastError());
     return OMB_OK;
 }

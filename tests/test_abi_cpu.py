@@ -72,6 +72,10 @@ def test_check_and_fill_reject_bad_arguments(ompblas):
     assert lib.omb_calib_fmac(4, 0, 16, None) == -1
     assert lib.omb_calib_fmac(4, 10, None, None) == -1
     assert "omb_calib_fmac" in ompblas.last_error()
+    assert lib.omb_calib_mfma(0, 10, 1, 16, None) == -1
+    assert lib.omb_calib_mfma(4, 0, 1, 16, None) == -1
+    assert lib.omb_calib_mfma(4, 10, 0, None, None) == -1
+    assert "omb_calib_mfma" in ompblas.last_error()
 
 
 def test_torch_mirror_shape_errors(ompblas):
