@@ -3,14 +3,17 @@
 // Counterpart of amt::mtm_helper / impl::mtm_kernel / simd_loop<MTM>
 // (include/mtm.hpp:83-206, include/simd_loop.hpp:80-192): the reference packs
 // MR x KC and KC x NR panels into cache-sized buffers and runs a 16x6 register
-// micro-kernel.  Here a 256-thread workgroup owns a 128x128 tile of C, stages
-// 128x16 slices of A and B through LDS (double buffered; the staging plays
-// the role of `pack`, including the layout change for either operand order),
-// and every thread keeps an 8x8 register micro-tile.  Global loads run two
-// slices ahead in registers and LDS fragments one k ahead.
+// micro-kernel.  Here a 256-thread workgroup owns a BM x BN tile of C (three
+// classes: 128x128, 128x64, 64x64 -- a3's cache-level block choice becomes a
+// choice of tile class and K split per shape), stages BM x 16 and 16 x BN
+// slices of A and B through LDS (double buffered; the staging plays the role
+// of `pack`, including the layout change for either operand order), and
+// every thread keeps a (BM/16) x (BN/16) register micro-tile.  Global loads
+// run two slices ahead in registers and LDS fragments one k ahead.
 //
 // Accumulation order: each thread sums 128 consecutive k (8 LDS slices) into
-// an inner register tile, then adds it to an outer tile.  This two-level sum
+// an inner register tile, then adds it to an outer tile -- the same for every
+// tile class, so the class never changes an output's bits.  This two-level sum
 // keeps the rounding error of long K at or below the reference's (which sums
 // KC consecutive k and then does C += block, simd_loop.hpp:120-128,160-190).
 //
@@ -21,8 +24,8 @@
 namespace omb {
 namespace {
 
-constexpr int BM = 128, BN = 128, BK = 16;
-constexpr int NT = 256;         // threads per workgroup
+constexpr int BK = 16;
+constexpr int NT = 256;         // threads per workgroup (a 16 x 16 thread grid)
 constexpr int INNER_K = 128;    // k steps per inner accumulation block
 constexpr int PAD = 4;          // LDS row padding (floats), keeps float4 alignment
 constexpr int kGroupM = 8;      // tile rows per L2 group
@@ -35,6 +38,14 @@ constexpr double kSplitMaxBytes = 128.0 * (1 << 20);  // most partial-sum bytes
 // 2048^3: one wave per SIMD loses the dual issue of two).
 constexpr double kTileRateF32 = 0.17e12;
 constexpr double kTileRateF32Alone = 0.2e12;
+// The same for the 128x64 class (three per CU) and the 64x64 class (four
+// per CU), scaled from one measurement of all three classes (8192 x 8192 x
+// 2048 for the full rate, 16 x 16 tiles with K = 4096 alone): per workgroup
+// 0.154 / 0.103 / 0.072 TFLOP/s full and 0.196 / 0.155 / 0.126 alone.
+constexpr double kTileRateF32_128x64 = 0.114e12;
+constexpr double kTileRateF32_128x64Alone = 0.158e12;
+constexpr double kTileRateF32_64x64 = 0.079e12;
+constexpr double kTileRateF32_64x64Alone = 0.129e12;
 constexpr double kSplitHbm = 4.0e12;      // bytes/s for the partials' traffic
 constexpr double kSplitLaunch = 4.0e-6;   // the reduction kernel's launch and tail
 constexpr double kSplitReduceStep = 1.0e-6;  // one round trip per 8 partials of an element
@@ -67,27 +78,32 @@ __device__ inline float4 load4(const float* p, int n) {
 
 __device__ inline int clamp4(int64_t r) { return r <= 0 ? 0 : (r >= 4 ? 4 : (int)r); }
 
-// Global -> registers for one BK slice.  Each thread loads two float4 of A
-// and two of B along the operand's unit-stride dimension.
-template <bool A_KC, bool B_NC, bool VEC>
+// Global -> registers for one BK slice of a BM x BN tile: LA = BM/64 float4
+// of A and LB = BN/64 float4 of B per thread, each along the operand's
+// unit-stride dimension.
+template <int BM, int BN, bool A_KC, bool B_NC, bool VEC>
 __device__ inline void load_slice(const Args& p, int64_t m0, int64_t n0, int64_t k0, int tid,
-                                  float4 (&ra)[2], float4 (&rb)[2]) {
+                                  float4 (&ra)[BM / 64], float4 (&rb)[BN / 64]) {
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        int e = tid + i * NT;
+    for (int i = 0; i < BM / 64; ++i) {
+        const int e = tid + i * NT;
         if (A_KC) {  // A[m*lda + k]: float4 along k
             int row = e >> 2, kq = (e & 3) * 4;
             int64_t m = m0 + row, k = k0 + kq;
             int n = m < p.M ? clamp4(p.K - k) : 0;
             ra[i] = load4<VEC>(p.A + (n ? m * p.lda + k : 0), n);
         } else {     // A[k*lda + m]: float4 along m
-            int mq = (e & 31) * 4, kk = e >> 5;
+            int mq = (e % (BM / 4)) * 4, kk = e / (BM / 4);
             int64_t m = m0 + mq, k = k0 + kk;
             int n = k < p.K ? clamp4(p.M - m) : 0;
             ra[i] = load4<VEC>(p.A + (n ? k * p.lda + m : 0), n);
         }
+    }
+#pragma unroll
+    for (int i = 0; i < BN / 64; ++i) {
+        const int e = tid + i * NT;
         if (B_NC) {  // B[k*ldb + n]: float4 along n
-            int nq = (e & 31) * 4, kk = e >> 5;
+            int nq = (e % (BN / 4)) * 4, kk = e / (BN / 4);
             int64_t nn = n0 + nq, k = k0 + kk;
             int n = k < p.K ? clamp4(p.N - nn) : 0;
             rb[i] = load4<VEC>(p.B + (n ? k * p.ldb + nn : 0), n);
@@ -101,12 +117,12 @@ __device__ inline void load_slice(const Args& p, int64_t m0, int64_t n0, int64_t
 }
 
 // Registers -> LDS, laid out [k][m] and [k][n].
-template <bool A_KC, bool B_NC>
+template <int BM, int BN, bool A_KC, bool B_NC>
 __device__ inline void store_slice(float (*As)[BM + PAD], float (*Bs)[BN + PAD], int tid,
-                                   const float4 (&ra)[2], const float4 (&rb)[2]) {
+                                   const float4 (&ra)[BM / 64], const float4 (&rb)[BN / 64]) {
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        int e = tid + i * NT;
+    for (int i = 0; i < BM / 64; ++i) {
+        const int e = tid + i * NT;
         if (A_KC) {
             int row = e >> 2, kq = (e & 3) * 4;
             As[kq + 0][row] = ra[i].x;
@@ -114,11 +130,15 @@ __device__ inline void store_slice(float (*As)[BM + PAD], float (*Bs)[BN + PAD],
             As[kq + 2][row] = ra[i].z;
             As[kq + 3][row] = ra[i].w;
         } else {
-            int mq = (e & 31) * 4, kk = e >> 5;
+            int mq = (e % (BM / 4)) * 4, kk = e / (BM / 4);
             *reinterpret_cast<float4*>(&As[kk][mq]) = ra[i];
         }
+    }
+#pragma unroll
+    for (int i = 0; i < BN / 64; ++i) {
+        const int e = tid + i * NT;
         if (B_NC) {
-            int nq = (e & 31) * 4, kk = e >> 5;
+            int nq = (e % (BN / 4)) * 4, kk = e / (BN / 4);
             *reinterpret_cast<float4*>(&Bs[kk][nq]) = rb[i];
         } else {
             int col = e >> 2, kq = (e & 3) * 4;
@@ -130,8 +150,18 @@ __device__ inline void store_slice(float (*As)[BM + PAD], float (*Bs)[BN + PAD],
     }
 }
 
-template <bool A_KC, bool B_NC, bool VEC>
-__global__ __launch_bounds__(NT, 2) void sgemm_f32_kernel(Args q) {
+// Workgroups per CU each class is built for: the 128x128 class holds 2 x 64
+// accumulators per thread (2 workgroups per CU), 128x64 2 x 32 (3), 64x64
+// 2 x 16 (4).
+constexpr int occ_of(int tile_elems) {
+    return tile_elems >= 128 * 128 ? 2 : (tile_elems >= 128 * 64 ? 3 : 4);
+}
+
+// Thread (ty, tx) of the 16 x 16 grid owns rows {g*64 + ty*4 + i} (g < GR =
+// BM/64, i < 4) and columns {h*64 + tx*4 + j} (h < GC = BN/64, j < 4).
+template <int BM, int BN, bool A_KC, bool B_NC, bool VEC>
+__global__ __launch_bounds__(NT, occ_of(BM * BN)) void sgemm_f32_kernel(Args q) {
+    constexpr int GR = BM / 64, GC = BN / 64, TR = 4 * GR, TC = 4 * GC;
     __shared__ float As[2][BK][BM + PAD];
     __shared__ float Bs[2][BK][BN + PAD];
 
@@ -155,75 +185,86 @@ __global__ __launch_bounds__(NT, 2) void sgemm_f32_kernel(Args q) {
     const int64_t m0 = tm * BM, n0 = tn * BN;
 
     const int tx = tid & 15, ty = tid >> 4;
-    float inner[8][8], outer[8][8];
+    float inner[TR][TC], outer[TR][TC];
 #pragma unroll
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i < TR; ++i)
 #pragma unroll
-        for (int j = 0; j < 8; ++j) inner[i][j] = outer[i][j] = 0.f;
+        for (int j = 0; j < TC; ++j) inner[i][j] = outer[i][j] = 0.f;
 
     const int64_t nk = ceil_div(p.K, BK);
     constexpr int kFold = INNER_K / BK;
     // Slices kt and kt+1 live in LDS buffers 0 and 1; global loads run two
     // slices ahead in two register sets.
     auto compute = [&](int cur, int64_t kt) {
-        float4 fa[2][2], fb[2][2];
+        float4 fa[2][GR], fb[2][GC];
         auto fragment = [&](int k, int set) {
-            fa[set][0] = *reinterpret_cast<const float4*>(&As[cur][k][ty * 4]);
-            fa[set][1] = *reinterpret_cast<const float4*>(&As[cur][k][64 + ty * 4]);
-            fb[set][0] = *reinterpret_cast<const float4*>(&Bs[cur][k][tx * 4]);
-            fb[set][1] = *reinterpret_cast<const float4*>(&Bs[cur][k][64 + tx * 4]);
+#pragma unroll
+            for (int g = 0; g < GR; ++g)
+                fa[set][g] = *reinterpret_cast<const float4*>(&As[cur][k][g * 64 + ty * 4]);
+#pragma unroll
+            for (int h = 0; h < GC; ++h)
+                fb[set][h] = *reinterpret_cast<const float4*>(&Bs[cur][k][h * 64 + tx * 4]);
         };
         fragment(0, 0);
 #pragma unroll
         for (int k = 0; k < BK; ++k) {
             if (k + 1 < BK) fragment(k + 1, (k + 1) & 1);
-            const float4 a0 = fa[k & 1][0], a1 = fa[k & 1][1];
-            const float4 b0 = fb[k & 1][0], b1 = fb[k & 1][1];
-            const float a[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
-            const float b[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+            float a[TR], b[TC];
 #pragma unroll
-            for (int i = 0; i < 8; ++i)
+            for (int g = 0; g < GR; ++g) {
+                const float4 v = fa[k & 1][g];
+                a[4 * g] = v.x; a[4 * g + 1] = v.y; a[4 * g + 2] = v.z; a[4 * g + 3] = v.w;
+            }
 #pragma unroll
-                for (int j = 0; j < 8; ++j) inner[i][j] = fmaf(a[i], b[j], inner[i][j]);
+            for (int h = 0; h < GC; ++h) {
+                const float4 v = fb[k & 1][h];
+                b[4 * h] = v.x; b[4 * h + 1] = v.y; b[4 * h + 2] = v.z; b[4 * h + 3] = v.w;
+            }
+#pragma unroll
+            for (int i = 0; i < TR; ++i)
+#pragma unroll
+                for (int j = 0; j < TC; ++j) inner[i][j] = fmaf(a[i], b[j], inner[i][j]);
         }
         if ((kt + 1) % kFold == 0 || kt + 1 == nk) {
 #pragma unroll
-            for (int i = 0; i < 8; ++i)
+            for (int i = 0; i < TR; ++i)
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
+                for (int j = 0; j < TC; ++j) {
                     outer[i][j] += inner[i][j];
                     inner[i][j] = 0.f;
                 }
         }
     };
-    float4 ra0[2], rb0[2], ra1[2], rb1[2];
-    load_slice<A_KC, B_NC, VEC>(p, m0, n0, 0, tid, ra0, rb0);
-    store_slice<A_KC, B_NC>(As[0], Bs[0], tid, ra0, rb0);
-    if (nk > 1) load_slice<A_KC, B_NC, VEC>(p, m0, n0, BK, tid, ra1, rb1);
+    float4 ra0[GR], rb0[GC], ra1[GR], rb1[GC];
+    load_slice<BM, BN, A_KC, B_NC, VEC>(p, m0, n0, 0, tid, ra0, rb0);
+    store_slice<BM, BN, A_KC, B_NC>(As[0], Bs[0], tid, ra0, rb0);
+    if (nk > 1) load_slice<BM, BN, A_KC, B_NC, VEC>(p, m0, n0, BK, tid, ra1, rb1);
     __syncthreads();
     for (int64_t kt = 0; kt < nk; kt += 2) {
-        if (kt + 2 < nk) load_slice<A_KC, B_NC, VEC>(p, m0, n0, (kt + 2) * BK, tid, ra0, rb0);
+        if (kt + 2 < nk)
+            load_slice<BM, BN, A_KC, B_NC, VEC>(p, m0, n0, (kt + 2) * BK, tid, ra0, rb0);
         compute(0, kt);
-        if (kt + 1 < nk) store_slice<A_KC, B_NC>(As[1], Bs[1], tid, ra1, rb1);
+        if (kt + 1 < nk) store_slice<BM, BN, A_KC, B_NC>(As[1], Bs[1], tid, ra1, rb1);
         __syncthreads();
         if (kt + 1 >= nk) break;
-        if (kt + 3 < nk) load_slice<A_KC, B_NC, VEC>(p, m0, n0, (kt + 3) * BK, tid, ra1, rb1);
+        if (kt + 3 < nk)
+            load_slice<BM, BN, A_KC, B_NC, VEC>(p, m0, n0, (kt + 3) * BK, tid, ra1, rb1);
         compute(1, kt + 1);
-        if (kt + 2 < nk) store_slice<A_KC, B_NC>(As[0], Bs[0], tid, ra0, rb0);
+        if (kt + 2 < nk) store_slice<BM, BN, A_KC, B_NC>(As[0], Bs[0], tid, ra0, rb0);
         __syncthreads();
     }
 
-    // Epilogue: rows {ty*4+i, 64+ty*4+i}, columns {tx*4+j, 64+tx*4+j}.
+    // Epilogue: rows {g*64 + ty*4 + i}, columns {h*64 + tx*4 + j}.
     if (p.part) {  // raw partial sums of this split, row-major M x N
         float* part = p.part + (int64_t)blockIdx.y * p.M * p.N;
         const bool pvec = p.N % 4 == 0;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int64_t m = m0 + (i < 4 ? ty * 4 + i : 64 + ty * 4 + (i - 4));
+        for (int i = 0; i < TR; ++i) {
+            const int64_t m = m0 + (i / 4) * 64 + ty * 4 + (i % 4);
             if (m >= p.M) continue;
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int64_t n = n0 + (h == 0 ? tx * 4 : 64 + tx * 4);
+            for (int h = 0; h < GC; ++h) {
+                const int64_t n = n0 + h * 64 + tx * 4;
                 float* c = part + m * p.N + n;
                 if (pvec && n + 3 < p.N) {
                     *reinterpret_cast<float4*>(c) = make_float4(outer[i][h * 4], outer[i][h * 4 + 1],
@@ -239,12 +280,12 @@ __global__ __launch_bounds__(NT, 2) void sgemm_f32_kernel(Args q) {
     }
     const bool cvec = ((reinterpret_cast<uintptr_t>(p.C) & 15) == 0) && (p.ldc % 4 == 0);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int64_t m = m0 + (i < 4 ? ty * 4 + i : 64 + ty * 4 + (i - 4));
+    for (int i = 0; i < TR; ++i) {
+        const int64_t m = m0 + (i / 4) * 64 + ty * 4 + (i % 4);
         if (m >= p.M) continue;
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int64_t n = n0 + (h == 0 ? tx * 4 : 64 + tx * 4);
+        for (int h = 0; h < GC; ++h) {
+            const int64_t n = n0 + h * 64 + tx * 4;
             float* c = p.C + m * p.ldc + n;
             float v[4];
 #pragma unroll
@@ -296,14 +337,34 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(int64_t M, int64_t N
     }
 }
 
-template <bool A_KC, bool B_NC>
+template <int BM, int BN, bool A_KC, bool B_NC>
 void launch_variant(const Args& a, int64_t splits, bool vec, hipStream_t s) {
     dim3 grid((unsigned)(a.tiles_m * a.tiles_n), (unsigned)splits), block(NT);
     if (vec)
-        hipLaunchKernelGGL((sgemm_f32_kernel<A_KC, B_NC, true>), grid, block, 0, s, a);
+        hipLaunchKernelGGL((sgemm_f32_kernel<BM, BN, A_KC, B_NC, true>), grid, block, 0, s, a);
     else
-        hipLaunchKernelGGL((sgemm_f32_kernel<A_KC, B_NC, false>), grid, block, 0, s, a);
+        hipLaunchKernelGGL((sgemm_f32_kernel<BM, BN, A_KC, B_NC, false>), grid, block, 0, s, a);
 }
+
+template <int BM, int BN>
+void launch_class(const Args& a, bool a_kc, bool b_nc, int64_t splits, bool vec, hipStream_t s) {
+    if (a_kc && b_nc) launch_variant<BM, BN, true, true>(a, splits, vec, s);
+    else if (a_kc && !b_nc) launch_variant<BM, BN, true, false>(a, splits, vec, s);
+    else if (!a_kc && b_nc) launch_variant<BM, BN, false, true>(a, splits, vec, s);
+    else launch_variant<BM, BN, false, false>(a, splits, vec, s);
+}
+
+// The plain tile classes: tile shape, workgroups per CU, and the FP32 flop/s
+// of one workgroup with the CU full of its class and alone on its CU.
+struct F32Class {
+    int bm, bn, occ;
+    double rate_full, rate_alone;
+};
+constexpr F32Class kF32Classes[3] = {
+    {128, 128, 2, kTileRateF32, kTileRateF32Alone},
+    {128, 64, 3, kTileRateF32_128x64, kTileRateF32_128x64Alone},
+    {64, 64, 4, kTileRateF32_64x64, kTileRateF32_64x64Alone},
+};
 
 __global__ void scale_kernel(int64_t M, int64_t N, float beta, float* C, int64_t ldc) {
     int64_t total = M * N;
@@ -317,25 +378,29 @@ __global__ void scale_kernel(int64_t M, int64_t N, float beta, float* C, int64_t
 
 }  // namespace
 
-int64_t gemm_k_splits(int64_t M, int64_t N, int64_t K, int64_t align, double rate_full,
-                      double rate_alone, int64_t* kchunk) {
+int64_t gemm_k_splits_tiles(int64_t M, int64_t N, int64_t K, int64_t align, int bm, int bn,
+                            int occ, double rate_full, double rate_alone, int64_t* kchunk,
+                            double* time) {
     *kchunk = K;
-    const int64_t tiles = ceil_div(M, 128) * ceil_div(N, 128);
+    const int64_t tiles = ceil_div(M, bm) * ceil_div(N, bn);
     const int64_t cus = device_cu_count();
-    if (tiles <= 0 || K <= align) return 1;
-    // Rounds of workgroups (two resident per CU) times the k each covers,
+    // Rounds of workgroups (`occ` resident per CU) times the k each covers,
     // at the rate of a workgroup sharing its CU or, when there are no more
     // workgroups than CUs, alone on it; plus the partials' write and read,
     // the reduction launch and its round trips.
     auto cost = [&](int64_t splits, int64_t kc) {
         const int64_t w = tiles * splits;
-        const double work = 2.0 * 128.0 * 128.0 * (double)kc;
-        double t = w <= cus ? work / rate_alone : (double)ceil_div(w, 2 * cus) * work / rate_full;
+        const double work = 2.0 * bm * bn * (double)kc;
+        double t = w <= cus ? work / rate_alone : (double)ceil_div(w, occ * cus) * work / rate_full;
         if (splits > 1)
             t += (double)(splits + 1) * (double)(M * N) * 4.0 / kSplitHbm + kSplitLaunch +
                  (double)ceil_div(splits - 1, 8) * kSplitReduceStep;
         return t;
     };
+    if (tiles <= 0 || K <= align) {
+        if (time) *time = tiles > 0 ? cost(1, K) : 0.0;
+        return 1;
+    }
     const double part_bytes = (double)(M * N) * 4.0;
     // Exact choices first: no split, or one accumulation block per split.
     // The in-order sum of single blocks is the unsplit kernel's own fold
@@ -352,7 +417,10 @@ int64_t gemm_k_splits(int64_t M, int64_t N, int64_t K, int64_t align, double rat
             *kchunk = align;
         }
     }
-    if (K <= kSplitExactK) return best;
+    if (K <= kSplitExactK) {
+        if (time) *time = best_t;
+        return best;
+    }
     // Coarser splits (at least 256 k each) round differently: one is taken
     // only when it is more than 10% faster than the exact choice.
     const double exact_t = best_t;
@@ -370,11 +438,42 @@ int64_t gemm_k_splits(int64_t M, int64_t N, int64_t K, int64_t align, double rat
         }
     }
     if (best == 1) *kchunk = K;
+    if (time) *time = best_t;
+    return best;
+}
+
+int64_t gemm_k_splits(int64_t M, int64_t N, int64_t K, int64_t align, double rate_full,
+                      double rate_alone, int64_t* kchunk) {
+    return gemm_k_splits_tiles(M, N, K, align, 128, 128, 2, rate_full, rate_alone, kchunk,
+                               nullptr);
+}
+
+// Tile class and K split of a plain GEMM: the class whose modelled time is
+// least (a smaller class only when it is 5% faster than the larger one
+// before it).  Any class and any exact split give the same bits.
+int sgemm_f32_choose(int64_t M, int64_t N, int64_t K, int64_t* splits, int64_t* kchunk) {
+    int best = 0;
+    double best_t = 0.0;
+    for (int c = 0; c < 3; ++c) {
+        const F32Class& k = kF32Classes[c];
+        int64_t kc = K;
+        double t = 0.0;
+        const int64_t sp = gemm_k_splits_tiles(M, N, K, INNER_K, k.bm, k.bn, k.occ, k.rate_full,
+                                               k.rate_alone, &kc, &t);
+        if (c == 0 || t < 0.95 * best_t) {
+            best = c;
+            best_t = t;
+            *splits = sp;
+            *kchunk = kc;
+        }
+    }
     return best;
 }
 
 int64_t sgemm_f32_splits(int64_t M, int64_t N, int64_t K, int64_t* kchunk) {
-    return gemm_k_splits(M, N, K, INNER_K, kTileRateF32, kTileRateF32Alone, kchunk);
+    int64_t splits = 1;
+    sgemm_f32_choose(M, N, K, &splits, kchunk);
+    return splits;
 }
 
 omb_status launch_splitk_reduce(int64_t M, int64_t N, int64_t splits, float alpha, float beta,
@@ -391,16 +490,18 @@ omb_status launch_sgemm_f32(int64_t M, int64_t N, int64_t K, float alpha,
                             const float* A, int64_t lda, bool a_kc,
                             const float* B, int64_t ldb, bool b_nc,
                             float beta, float* C, int64_t ldc, hipStream_t s, float* part) {
-    Args a{M, N, K, alpha, beta, A, lda, B, ldb, C, ldc, ceil_div(M, BM), ceil_div(N, BN), K,
-           nullptr};
+    // Tile class per shape; split K over blockIdx.y when the tiles alone
+    // leave CUs idle (the splits' partials are then added in split order).
+    int64_t splits = 1, kchunk = K;
+    const int cls = sgemm_f32_choose(M, N, K, &splits, &kchunk);
+    const F32Class& fc = kF32Classes[cls];
+    Args a{M, N, K, alpha, beta, A, lda, B, ldb, C, ldc, ceil_div(M, fc.bm), ceil_div(N, fc.bn),
+           kchunk, nullptr};
     const int64_t tiles = a.tiles_m * a.tiles_n;
     if (tiles > 0x7fffffffLL) {
         set_error("sgemm: too many tiles for one launch");
         return OMB_EUNSUPPORTED;
     }
-    // Split K over blockIdx.y when the tiles alone leave CUs idle; the
-    // splits' partials are then added in split order.
-    const int64_t splits = sgemm_f32_splits(M, N, K, &a.kchunk);
     if (splits > 1) {
         if (!part) {
             Workspace* ws = nullptr;
@@ -412,10 +513,9 @@ omb_status launch_sgemm_f32(int64_t M, int64_t N, int64_t K, float alpha,
     const bool vec = ((reinterpret_cast<uintptr_t>(A) & 15) == 0) &&
                      ((reinterpret_cast<uintptr_t>(B) & 15) == 0) && (lda % 4 == 0) &&
                      (ldb % 4 == 0);
-    if (a_kc && b_nc) launch_variant<true, true>(a, splits, vec, s);
-    else if (a_kc && !b_nc) launch_variant<true, false>(a, splits, vec, s);
-    else if (!a_kc && b_nc) launch_variant<false, true>(a, splits, vec, s);
-    else launch_variant<false, false>(a, splits, vec, s);
+    if (cls == 0) launch_class<128, 128>(a, a_kc, b_nc, splits, vec, s);
+    else if (cls == 1) launch_class<128, 64>(a, a_kc, b_nc, splits, vec, s);
+    else launch_class<64, 64>(a, a_kc, b_nc, splits, vec, s);
     OMB_HIP_TRY(hipGetLastError());
     if (splits > 1) return launch_splitk_reduce(M, N, splits, alpha, beta, a.part, C, ldc, s);
     return OMB_OK;

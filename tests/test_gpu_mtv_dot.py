@@ -108,6 +108,35 @@ def test_gemv_deterministic_and_beta(dev, ompblas):
     assert torch.allclose(outs[0].double(), want, rtol=0, atol=1e-3)
 
 
+def test_gemv_column_major_single_pass_rows(dev, ompblas):
+    """Column-major A of whole 64-row blocks (at least one per CU) and whole
+    256-column rounds runs one pass of 64-row workgroups; a ragged shape of
+    the same size runs the column-chunked kernel.  An offset start, lda > M
+    and beta != 0, on integer values: exact against int64, and the same bits
+    at every alignment."""
+    cus = torch.cuda.get_device_properties(dev).multi_processor_count
+    for M, N in ((64 * cus, 256), (64 * cus + 64, 512), (64 * cus + 37, 301), (64 * cus, 257)):
+        ys = []
+        y0 = rand100((M,))
+        x = torch.from_numpy(rand100((N,))).to(dev)
+        a_np = rand100((M, N), "F")
+        for off, pad in ((0, 0), (1, 3), (3, 1)):
+            ld = M + pad
+            buf = torch.zeros(off + N * ld, device=dev)
+            view = buf[off:].view(N, ld)[:, :M]           # column-major M x N
+            view.copy_(torch.from_numpy(np.ascontiguousarray(a_np.T)).to(dev))
+            a = view.t()
+            helpers.assert_view_in_bounds(a, M, N, a.stride(0), a.stride(1))
+            y = torch.from_numpy(y0).to(dev)
+            ompblas.check(ompblas.sgemv(M, N, 1.0, a.data_ptr(), a.stride(0), a.stride(1),
+                                        x.data_ptr(), 3.0, y.data_ptr()))
+            ys.append(y)
+        want = a_np.astype(np.int64) @ x.cpu().numpy().astype(np.int64) + 3 * y0.astype(np.int64)
+        assert np.array_equal(ys[0].cpu().numpy().astype(np.int64), want), (M, N)
+        for y in ys[1:]:
+            assert torch.equal(y, ys[0]), (M, N)
+
+
 # ----------------------------------------------------------------- dot ----
 DOT_SIZES = list(range(2, 200)) + list(range(200, (1 << 17) + 2, 1009)) + [(1 << 17) + 1]
 
