@@ -97,7 +97,8 @@ def main():
                     help="seconds of host time for the CPU baseline sample")
     ap.add_argument("--legs", default="all",
                     help="comma list of legs beside the headline: plain, errors, gemv, dot, "
-                         "level2, grouped, serving, c1, cpu (default all)")
+                         "level2, grouped, serving, c1, cpu (default all); grouped_plans "
+                         "times the grouped plans only (no per-instance calls)")
     ap.add_argument("--serve-rate", type=float, default=float(os.environ.get("OMB_SERVE_RATE", 32000)),
                     help="serving leg: Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-count", type=int, default=int(os.environ.get("OMB_SERVE_COUNT", 3000)),
@@ -430,8 +431,9 @@ def main():
 
     # ------------------------------------------------ grouped and serving --
     grouped, serving = None, None
-    if "grouped" in legs:
-        grouped = grouped_leg(pkg, lib, torch, dev, stream, timer, args.steps, "cpu" in legs)
+    if "grouped" in legs or "grouped_plans" in legs:
+        grouped = grouped_leg(pkg, lib, torch, dev, stream, timer, args.steps, "cpu" in legs,
+                              per_instance_calls="grouped" in legs)
         sync()
     if "serving" in legs:
         serving = serving_leg(pkg, torch, dev, stream, args.serve_rate, args.serve_count,
@@ -583,7 +585,8 @@ def _pool(torch, dev, stream, sizes):
     return buf, offs
 
 
-def grouped_leg(pkg, lib, torch, dev, stream, timer, steps, cpu_sample=True):
+def grouped_leg(pkg, lib, torch, dev, stream, timer, steps, cpu_sample=True,
+                per_instance_calls=True):
     """4096 row-major GEMMs (M, N, K log-uniform on [16, 512]) and 4096 GEMVs
     (rows, cols log-uniform on [64, 4096]) from seed 3, operands U[-1,1),
     C = A*B and y = A*x, in one persistent grouped launch replayed from a
@@ -662,9 +665,11 @@ def grouped_leg(pkg, lib, torch, dev, stream, timer, steps, cpu_sample=True):
             m, n, a, lda, x, y, al, be = d
             lib.omb_sgemv(m, n, al, a, lda, 1, x, be, y, sp)
 
-    per_instance()
-    out["per_instance_calls_ms"] = timer.run(per_instance, 2) * 1e3
-    out["speedup_vs_per_instance_calls"] = out["per_instance_calls_ms"] / out["plain_f32"]["ms"]
+    if per_instance_calls:
+        per_instance()
+        out["per_instance_calls_ms"] = timer.run(per_instance, 2) * 1e3
+        out["speedup_vs_per_instance_calls"] = (out["per_instance_calls_ms"] /
+                                                out["plain_f32"]["ms"])
     if cpu_sample:
         out["cpu_sequential_sample"] = _grouped_cpu_sample(torch, pools, M, N, K, VM, VN)
     del pools

@@ -176,6 +176,28 @@ __device__ inline float4 ld4u(const float* p) {
     f32x4_u v = *reinterpret_cast<const f32x4_u*>(p);
     return make_float4(v.x, v.y, v.z, v.w);
 }
+// Global-address-space views of pointers that arrive through memory (work
+// descriptors): without them the compiler must emit flat loads, which also
+// count against the LDS wait counter, so every LDS wait drains the global
+// loads in flight.  Only for pointers into device (global) memory.
+#define OMB_GLOBAL __attribute__((address_space(1)))
+typedef const OMB_GLOBAL float gcfloat;
+typedef OMB_GLOBAL float gfloat;
+typedef const OMB_GLOBAL int gcint;
+typedef OMB_GLOBAL int gint;
+__device__ inline gcfloat* as_global(const float* p) { return (gcfloat*)p; }
+__device__ inline gfloat* as_global(float* p) { return (gfloat*)p; }
+__device__ inline gcint* as_global(const int* p) { return (gcint*)p; }
+__device__ inline gint* as_global(int* p) { return (gint*)p; }
+typedef const OMB_GLOBAL f32x4_u gcf32x4_u;
+__device__ inline float4 ld4u_nt(gcfloat* p) {
+    f32x4_u v = __builtin_nontemporal_load(reinterpret_cast<gcf32x4_u*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ inline float4 ld4u(gcfloat* p) {
+    f32x4_u v = *reinterpret_cast<gcf32x4_u*>(p);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
 __device__ inline float dot4(float4 a, float4 b, float acc) {
     acc = fmaf(a.x, b.x, acc);
     acc = fmaf(a.y, b.y, acc);

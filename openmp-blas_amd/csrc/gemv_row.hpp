@@ -12,13 +12,17 @@
 
 namespace omb {
 
-template <int ROWS, int UNROLL>
+// (PA / PX / PY: generic or global-address-space pointers.  STREAM: rows of A
+// are loaded with the non-temporal hint; the grouped kernel, whose items
+// are short and share cache lines across item boundaries, loads them
+// normally -- measured 3.5% faster there.)
+template <int ROWS, int UNROLL, bool STREAM, typename PA, typename PX, typename PY>
 __device__ __forceinline__ void gemv_rows_wave(int64_t r0, int nr, int64_t N, float alpha,
-                                               const float* __restrict__ A, int64_t lda,
-                                               const float* __restrict__ x, float beta,
-                                               float* __restrict__ y, int lane) {
+                                               PA __restrict__ A, int64_t lda,
+                                               PX __restrict__ x, float beta,
+                                               PY __restrict__ y, int lane) {
     float acc[ROWS];
-    const float* ar[ROWS];
+    PA ar[ROWS];
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
         acc[r] = 0.f;
@@ -34,7 +38,10 @@ __device__ __forceinline__ void gemv_rows_wave(int64_t r0, int nr, int64_t N, fl
 #pragma unroll
         for (int r = 0; r < ROWS; ++r)
 #pragma unroll
-            for (int u = 0; u < UNROLL; ++u) av[r][u] = ld4u_nt(ar[r] + 4 * (j + 64 * u));
+            for (int u = 0; u < UNROLL; ++u) {
+                const auto p = ar[r] + 4 * (j + 64 * u);
+                av[r][u] = STREAM ? ld4u_nt(p) : ld4u(p);
+            }
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u)
 #pragma unroll
@@ -43,7 +50,10 @@ __device__ __forceinline__ void gemv_rows_wave(int64_t r0, int nr, int64_t N, fl
     for (; j < n4; j += 64) {
         const float4 xv = ld4u(x + 4 * j);
 #pragma unroll
-        for (int r = 0; r < ROWS; ++r) acc[r] = dot4(ld4u_nt(ar[r] + 4 * j), xv, acc[r]);
+        for (int r = 0; r < ROWS; ++r) {
+            const auto p = ar[r] + 4 * j;
+            acc[r] = dot4(STREAM ? ld4u_nt(p) : ld4u(p), xv, acc[r]);
+        }
     }
     const int64_t t = (n4 << 2) + lane;
     if (t < N) {

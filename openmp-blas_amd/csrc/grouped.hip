@@ -98,7 +98,7 @@ constexpr int kFoldK = 128;  // k per inner accumulation block (as sgemm_f32)
 // consecutive columns (FR = min(TMR, 4), GR = TMR / FR; likewise for
 // columns), so every LDS read is one 64- or 128-bit access.  Slices of 16 k
 // are loaded with 128-bit global loads along the unit-stride index, staged
-// through LDS, and the next slice is prefetched into registers; LDS
+// through two LDS buffers, and the next slice is prefetched into registers; LDS
 // fragments for k+1 are read while k's FMAs run.  Every element sums k in
 // order and folds the running sum every 128 k, as sgemm_f32_kernel does, so
 // the results are the same bits.
@@ -113,7 +113,8 @@ struct FragT<4> {
     typedef float4 type;
 };
 
-__device__ __forceinline__ float4 load4_guard(const float* p, int n) {
+template <typename P>
+__device__ __forceinline__ float4 load4_guard(P p, int n) {
     if (n == 4) return ld4u(p);
     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
     if (n > 0) v.x = p[0];
@@ -137,11 +138,15 @@ __device__ __forceinline__ void tile_f32(const DGemm& g, int tm, int tn, float* 
     constexpr int LA = (NA4 + NT - 1) / NT, LB = (NB4 + NT - 1) / NT;
     typedef typename FragT<FR>::type FragA;
     typedef typename FragT<FC>::type FragB;
-    float* As = smem;            // [BK][PA]
-    float* Bs = smem + BK * PA;  // [BK][PB]
+    // Two LDS buffers of [BK][PA] A and [BK][PB] B: slice k+1 is stored into
+    // one while slice k is read from the other (one barrier per slice).
+    constexpr int SZ = BK * (PA + PB);
     const int tid = threadIdx.x, tx = tid % W, ty = tid / W;
     const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
     const int64_t M = g.M, N = g.N, K = g.K;
+    gcfloat* const gA = as_global(g.A);
+    gcfloat* const gB = as_global(g.B);
+    gfloat* const gC = as_global(g.C);
 
     float4 ra[LA], rb[LB];
     auto load = [&](int64_t k0) {
@@ -152,7 +157,7 @@ __device__ __forceinline__ void tile_f32(const DGemm& g, int tm, int tn, float* 
                 const int row = e >> 2, kq = (e & 3) * 4;
                 const int64_t m = m0 + row, k = k0 + kq;
                 const int n = m < M ? clamp4i(K - k) : 0;
-                ra[i] = load4_guard(g.A + (n ? m * g.lda + k : 0), n);
+                ra[i] = load4_guard(gA + (n ? m * g.lda + k : 0), n);
             }
         }
 #pragma unroll
@@ -162,11 +167,13 @@ __device__ __forceinline__ void tile_f32(const DGemm& g, int tm, int tn, float* 
                 const int kk = e / (BN / 4), nq = (e % (BN / 4)) * 4;
                 const int64_t nn = n0 + nq, k = k0 + kk;
                 const int n = k < K ? clamp4i(N - nn) : 0;
-                rb[i] = load4_guard(g.B + (n ? k * g.ldb + nn : 0), n);
+                rb[i] = load4_guard(gB + (n ? k * g.ldb + nn : 0), n);
             }
         }
     };
-    auto store = [&]() {
+    auto store = [&](int buf) {
+        float* As = smem + buf * SZ;
+        float* Bs = As + BK * PA;
 #pragma unroll
         for (int i = 0; i < LA; ++i) {
             const int e = tid + i * NT;
@@ -195,12 +202,9 @@ __device__ __forceinline__ void tile_f32(const DGemm& g, int tm, int tn, float* 
         for (int j = 0; j < TNR; ++j) inner[i][j] = outer[i][j] = 0.f;
 
     const int64_t nk = ceil_div(K, BK);
-    load(0);
-    for (int64_t kt = 0; kt < nk; ++kt) {
-        __syncthreads();  // the previous slice (or item) is done with LDS
-        store();
-        __syncthreads();
-        if (kt + 1 < nk) load((kt + 1) * BK);
+    auto compute = [&](int buf, int64_t kt) {
+        const float* As = smem + buf * SZ;
+        const float* Bs = As + BK * PA;
         FragA fa[2][GR];
         FragB fb[2][GC];
         auto fragment = [&](int k, int set) {
@@ -231,6 +235,18 @@ __device__ __forceinline__ void tile_f32(const DGemm& g, int tm, int tn, float* 
                     inner[i][j] = 0.f;
                 }
         }
+    };
+    load(0);
+    __syncthreads();  // the previous item is done with LDS
+    store(0);
+    __syncthreads();
+    for (int64_t kt = 0; kt < nk; ++kt) {
+        if (kt + 1 < nk) load((kt + 1) * BK);
+        compute((int)(kt & 1), kt);
+        // The other buffer was last read in step kt - 1, which every thread
+        // finished before the barrier that ended it.
+        if (kt + 1 < nk) store((int)((kt + 1) & 1));
+        __syncthreads();
     }
 #pragma unroll
     for (int i = 0; i < TMR; ++i) {
@@ -240,7 +256,7 @@ __device__ __forceinline__ void tile_f32(const DGemm& g, int tm, int tn, float* 
         for (int j = 0; j < TNR; ++j) {
             const int64_t n = n0 + (j / FC) * (BN / GC) + tx * FC + (j % FC);
             if (n < N) {
-                float* c = g.C + m * g.ldc + n;
+                gfloat* c = gC + m * g.ldc + n;
                 const float v = g.alpha * outer[i][j];
                 *c = g.beta != 0.f ? v + g.beta * *c : v;
             }
@@ -257,8 +273,9 @@ __device__ __forceinline__ void tile_f32(const DGemm& g, int tm, int tn, float* 
 // range read as 0.  Edge loads read a clamped in-range address and select,
 // so every load of the slice is in flight at once (A and B are non-null
 // whenever a tile has K > 0).
-__device__ __forceinline__ void load_split_slice(const float* A, int64_t lda, int64_t M,
-                                                 const float* B, int64_t ldb, int64_t N,
+template <typename P>
+__device__ __forceinline__ void load_split_slice(P A, int64_t lda, int64_t M,
+                                                 P B, int64_t ldb, int64_t N,
                                                  int64_t K, int64_t m, int64_t ka, int64_t n,
                                                  int64_t kb, float (&va)[4], float (&vb)[4]) {
     if (m < M && ka + 3 < K) {
@@ -289,8 +306,8 @@ __device__ __forceinline__ void tile_split(const DGemm& g, int tm, int tn, uint4
     const int frow = lane & 31, fh = lane >> 5;
     const int64_t m0 = (int64_t)tm * P, n0 = (int64_t)tn * P;
     const int64_t M = g.M, N = g.N, K = g.K;
-    const float* const A = g.A;
-    const float* const B = g.B;
+    gcfloat* const A = as_global(g.A);
+    gcfloat* const B = as_global(g.B);
     const int64_t lda = g.lda, ldb = g.ldb;
     // loader roles: A row ra, k quarter qa; B^T row rb (= column of B), k quarter qb
     const int ra = tid >> 2, qa = tid & 3;
@@ -328,7 +345,8 @@ __device__ __forceinline__ void tile_split(const DGemm& g, int tm, int tn, uint4
         mfma_slice<NPROD, 1, 1, P, P>(smem, wm * 32, wn * 32, frow, fh, inner);
         if ((kt + 1) % kFoldSlices == 0 || kt + 1 == nkt) fold(inner, outer);
     }
-    store_tile(outer, m0 + wm * 32, n0 + wn * 32, lane, g.C, g.ldc, M, N, g.alpha, g.beta);
+    store_tile(outer, m0 + wm * 32, n0 + wn * 32, lane, as_global(g.C), g.ldc, M, N, g.alpha,
+               g.beta);
 }
 
 // --------------------------------------------------- scaled FP16 split --
@@ -398,8 +416,8 @@ __device__ __forceinline__ void tile_split16(const DGemm& g, int tm, int tn, uin
     const int frow = lane & 31, fh = lane >> 5;
     const int64_t m0 = (int64_t)tm * P, n0 = (int64_t)tn * P;
     const int64_t M = g.M, N = g.N, K = g.K;
-    const float* const A = g.A;
-    const float* const B = g.B;
+    gcfloat* const A = as_global(g.A);
+    gcfloat* const B = as_global(g.B);
     const int64_t lda = g.lda, ldb = g.ldb;
     const int ra = tid >> 2, qa = tid & 3;
     const int rb = tid & 63, qb = tid >> 6;
@@ -440,8 +458,9 @@ __device__ __forceinline__ void tile_split16(const DGemm& g, int tm, int tn, uin
         mfma_slice<3, 1, 1, P, P, f16x8, 2>(smem, wm * 32, wn * 32, frow, fh, inner);
         if ((kt + 1) % kFoldSlices == 0 || kt + 1 == nkt) fold(inner, outer);
     }
-    store_tile(outer, m0 + wm * 32, n0 + wn * 32, lane, g.C, g.ldc, M, N, g.alpha, g.beta, g.ea,
-               g.eb);
+    store_tile(outer, m0 + wm * 32, n0 + wn * 32, lane, as_global(g.C), g.ldc, M, N, g.alpha,
+               g.beta, as_global(static_cast<const int*>(g.ea)),
+               as_global(static_cast<const int*>(g.eb)));
 }
 
 // ------------------------------------------------------------- GEMV item --
@@ -452,13 +471,14 @@ __device__ __forceinline__ void gemv_item(const DGemv& v, int chunk) {
     const int64_t r0 = (int64_t)chunk * 64 + wave * 16;
     if (r0 >= v.M) return;
     const int nr = (int)(v.M - r0 < 16 ? v.M - r0 : 16);
-    gemv_rows_wave<16, 1>(r0, nr, v.N, v.alpha, v.A, v.lda, v.x, v.beta, v.y, lane);
+    gemv_rows_wave<16, 1, false>(r0, nr, v.N, v.alpha, as_global(v.A), v.lda, as_global(v.x), v.beta,
+                          as_global(v.y), lane);
 }
 
 // ------------------------------------------------------- persistent kernel --
-// LDS: the largest plain class footprint (128x64: 16 x (132 + 68) floats);
-// the split tile uses 12 KB.
-constexpr int kSmemFloats = 16 * (128 + 4) + 16 * (64 + 4) + 16;
+// LDS: the largest plain class footprint (128x64: two buffers of
+// 16 x (132 + 68) floats); the split tile uses 12 KB.
+constexpr int kSmemFloats = 2 * (16 * (128 + 4) + 16 * (64 + 4)) + 16;
 
 // The lite build is bounded to 3 workgroups per CU (at most 168 VGPRs), the
 // big one (128x64 tiles) to 2.

@@ -41,7 +41,7 @@ __global__ __launch_bounds__(NT) void sgemv_row_kernel(int64_t M, int64_t N, flo
          w * ROWS < M; w += waves) {
         const int64_t r0 = w * ROWS;
         const int nr = (int)std::min<int64_t>(ROWS, M - r0);
-        gemv_rows_wave<ROWS, UNROLL>(r0, nr, N, alpha, A, lda, x, beta, y, lane);
+        gemv_rows_wave<ROWS, UNROLL, true>(r0, nr, N, alpha, A, lda, x, beta, y, lane);
     }
 }
 

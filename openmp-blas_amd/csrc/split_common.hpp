@@ -148,11 +148,11 @@ __device__ __forceinline__ void fold(f32x16 (&inner)[BI][BJ], f32x16 (&outer)[BI
 // operands: row m of A was scaled by 2^ea[m], column n of B by 2^eb[n]),
 // acc is first scaled back by 2^-(ea[m] + eb[n]), exactly unless the result
 // leaves FP32's normal range (as the FP32 product itself would).
-template <int BI, int BJ>
+template <int BI, int BJ, typename PC, typename PE = const int*>
 __device__ __forceinline__ void store_tile(const f32x16 (&acc)[BI][BJ], int64_t m_base,
-                                           int64_t n_base, int lane, float* C, int64_t ldc,
+                                           int64_t n_base, int lane, PC C, int64_t ldc,
                                            int64_t M, int64_t N, float alpha, float beta,
-                                           const int* ea = nullptr, const int* eb = nullptr) {
+                                           PE ea = nullptr, PE eb = nullptr) {
 #pragma unroll
     for (int i = 0; i < BI; ++i)
 #pragma unroll
@@ -164,7 +164,7 @@ __device__ __forceinline__ void store_tile(const f32x16 (&acc)[BI][BJ], int64_t 
             for (int r = 0; r < 16; ++r) {
                 const int64_t m = m_base + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
                 if (m < M) {
-                    float* c = C + m * ldc + n;
+                    const PC c = C + m * ldc + n;
                     const float a = ea ? ldexpf(acc[i][j][r], -(ea[m] + en)) : acc[i][j][r];
                     const float v = alpha * a;
                     *c = beta != 0.f ? v + beta * *c : v;
