@@ -756,21 +756,29 @@ def serving_leg(pkg, torch, dev, stream, rate, count, slots):
            "offered_tflops": st["total_flop"] / (trace[-1][4] or 1.0) / 1e12,
            "operand_gbps": busy_bytes / st["wall_s"] / 1e9,
            "max_in_flight": st["max_in_flight"], "admitted_late": st["admitted_late"],
+           "batches": st["batches"], "batched_requests": st["batched_requests"],
            "p99_ms_by_kind": {name: float(np.percentile(lat[kinds == k], 99)) * 1e3
                               for k, name in ((0, "gemm"), (1, "gemv"), (2, "dot"))
                               if (kinds == k).any()},
            "service_p99_ms": float(np.percentile(svc, 99)) * 1e3}
-    sweep = []
-    for r in (8000.0, 16000.0, 48000.0, 64000.0, 96000.0):
-        tr = pkg.serving_trace(count, r, seed=4)
-        reqs = requests()
-        for q, t in zip(reqs, tr):
-            q.arrival_s = t[4]
-        _, s2 = pkg.serve_run(reqs, slots=slots)
-        sweep.append({"rate_per_s": r, "p50_ms": s2["p50_s"] * 1e3, "p99_ms": s2["p99_s"] * 1e3,
-                      "aggregate_tflops": s2["total_flop"] / s2["wall_s"] / 1e12})
-    out["rate_sweep"] = sweep
-    res1, st1 = pkg.serve_run(requests(), slots=1)
+    def sweep_at(rates, batching):
+        rows = []
+        for r in rates:
+            tr = pkg.serving_trace(count, r, seed=4)
+            reqs = requests()
+            for q, t in zip(reqs, tr):
+                q.arrival_s = t[4]
+            _, s2 = pkg.serve_run(reqs, slots=slots, batching=batching)
+            rows.append({"rate_per_s": r, "p50_ms": s2["p50_s"] * 1e3,
+                         "p99_ms": s2["p99_s"] * 1e3,
+                         "aggregate_tflops": s2["total_flop"] / s2["wall_s"] / 1e12,
+                         "batches": s2["batches"], "batched_requests": s2["batched_requests"]})
+        return rows
+
+    out["rate_sweep"] = sweep_at((8000.0, 16000.0, 48000.0, 64000.0, 96000.0), True)
+    # the same trace with a stream per request (no grouped launches)
+    out["rate_sweep_unbatched"] = sweep_at((32000.0, 64000.0, 96000.0), False)
+    res1, st1 = pkg.serve_run(requests(), slots=1, batching=False)
     out["fifo_one_slot"] = {"p50_ms": st1["p50_s"] * 1e3, "p99_ms": st1["p99_s"] * 1e3,
                             "aggregate_tflops": st1["total_flop"] / st1["wall_s"] / 1e12}
     del pools

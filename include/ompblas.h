@@ -257,6 +257,9 @@ typedef struct {
     int64_t hbm_budget;     /* bytes of workspace the loop may hold at once (0 = free HBM - reserve) */
     int64_t hbm_reserve;    /* bytes kept free when hbm_budget == 0 */
     int32_t spin_poll;      /* 1 = busy-poll completions, 0 = yield between polls */
+    int32_t no_batching;    /* 0 = while at least `slots` requests wait, small GEMM / GEMV
+                             * requests eligible together (two or more) share one grouped
+                             * launch on one slot; 1 = a stream per request */
 } omb_serve_config;
 
 typedef struct {
@@ -273,6 +276,8 @@ typedef struct {
     double total_flop;      /* sum of 2MNK / 2MN / 2n */
     int32_t max_in_flight;  /* most requests observed on the GPU at once */
     int32_t admitted_late;  /* requests that waited for a slot or for memory */
+    int32_t batches;        /* grouped launches of small requests */
+    int32_t batched_requests; /* requests that ran in them */
 } omb_serve_stats;
 
 omb_status omb_serve_run(omb_request* reqs, int32_t n, const omb_serve_config* cfg,

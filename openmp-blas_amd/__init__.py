@@ -83,7 +83,8 @@ class Request(ctypes.Structure):
 
 class ServeConfig(ctypes.Structure):
     _fields_ = [("slots", ctypes.c_int32), ("hbm_budget", ctypes.c_int64),
-                ("hbm_reserve", ctypes.c_int64), ("spin_poll", ctypes.c_int32)]
+                ("hbm_reserve", ctypes.c_int64), ("spin_poll", ctypes.c_int32),
+                ("no_batching", ctypes.c_int32)]
 
 
 class RequestResult(ctypes.Structure):
@@ -97,7 +98,8 @@ class ServeStats(ctypes.Structure):
                 ("p90_s", ctypes.c_double), ("p99_s", ctypes.c_double),
                 ("max_s", ctypes.c_double), ("mean_s", ctypes.c_double),
                 ("total_flop", ctypes.c_double), ("max_in_flight", ctypes.c_int32),
-                ("admitted_late", ctypes.c_int32)]
+                ("admitted_late", ctypes.c_int32), ("batches", ctypes.c_int32),
+                ("batched_requests", ctypes.c_int32)]
 
 
 _I64, _F32, _VP, _I32, _U64, _F64 = (ctypes.c_int64, ctypes.c_float, ctypes.c_void_p,
@@ -459,7 +461,7 @@ def autotune_load(table_path: str) -> None:
 
 # ---------------------------------------------------------- serving loop --
 def serve_run(requests, slots: int = 8, hbm_budget: int = 0, hbm_reserve: int = 1 << 30,
-              spin_poll: bool = True):
+              spin_poll: bool = True, batching: bool = True):
     """Replay `requests` (a list of Request) through omb_serve_run.
 
     Returns (results, stats): one dict per request (start_s, done_s,
@@ -470,7 +472,8 @@ def serve_run(requests, slots: int = 8, hbm_budget: int = 0, hbm_reserve: int = 
     reqs = (Request * max(1, n))(*requests)
     res = (RequestResult * max(1, n))()
     st = ServeStats()
-    cfg = ServeConfig(slots, hbm_budget, hbm_reserve, 1 if spin_poll else 0)
+    cfg = ServeConfig(slots, hbm_budget, hbm_reserve, 1 if spin_poll else 0,
+                      0 if batching else 1)
     check(lib().omb_serve_run(reqs, n, ctypes.byref(cfg), res, ctypes.byref(st)))
     out = []
     for i in range(n):

@@ -147,6 +147,19 @@ omb_status launch_sdot_noalloc(int64_t n, const float* x, const float* y, float*
 omb_status launch_fill_uniform(float* dst, int64_t n, uint64_t seed, uint64_t stream_id,
                                int64_t offset, hipStream_t s);
 
+// Grouped launches without a plan object (the serving loop's batches): a
+// device block for up to 1024 GEMMs, 1024 GEMVs and 65536 items, with its
+// pinned staging block and counters.  batch_launch builds the items on the
+// host, copies them on stream s and runs the persistent kernel; the caller
+// launches again from the same batch only after the previous launch has
+// completed.  Plain and split-BF16 (6 or 3 products) modes.
+struct GroupBatch;
+omb_status batch_create(GroupBatch** out);
+void batch_destroy(GroupBatch* b);
+bool batch_mode_supported(int mode);
+omb_status batch_launch(GroupBatch* b, const omb_gemm_desc* gemms, int n_gemm,
+                        const omb_gemv_desc* gemvs, int n_gemv, int mode, hipStream_t s);
+
 // ----------------------------------------------------- device helpers -----
 #if defined(__HIPCC__)
 // Counter-based generator (splitmix64, Steele/Lea/Flood 2014).

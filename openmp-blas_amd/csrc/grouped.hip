@@ -1035,3 +1035,148 @@ extern "C" omb_status omb_autotune_load(const char* table_path) {
     for (auto& kv : loaded) tune_table()[kv.first] = kv.second;
     return OMB_OK;
 }
+
+// ---------------------------------------------------------- request batches --
+// A grouped launch without a plan object, for the serving loop: the
+// descriptors and the cost-sorted item list are built on the host into a
+// pinned staging block, copied to the batch's device block on the launch
+// stream, and the lite persistent kernel runs them.  Results are bitwise
+// those of the grouped plans (and so of the direct calls, see above).  The
+// staging block is reused only after the previous launch from it has
+// completed, which the caller guarantees (one batch per slot at a time).
+namespace omb {
+
+struct GroupBatch {
+    char* dev = nullptr;
+    char* host = nullptr;
+    unsigned* counters = nullptr;
+    size_t bytes = 0;
+};
+
+namespace {
+constexpr int kBatchMaxGemms = 1024, kBatchMaxGemvs = 1024, kBatchMaxItems = 65536;
+
+size_t batch_block_bytes() {
+    return round_up(sizeof(DGemm) * kBatchMaxGemms, 256) +
+           round_up(sizeof(DGemv) * kBatchMaxGemvs, 256) + sizeof(Item) * kBatchMaxItems;
+}
+
+int lite_per_cu() {
+    static int per_cu = -1;
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lock(mu);
+    if (per_cu < 0) {
+        int v = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, grouped_kernel<false>, NT, 0) !=
+                hipSuccess ||
+            v < 1) {
+            (void)hipGetLastError();
+            v = 1;
+        }
+        per_cu = v;
+    }
+    return per_cu;
+}
+}  // namespace
+
+omb_status batch_create(GroupBatch** out) {
+    *out = nullptr;
+    auto* b = new GroupBatch();
+    b->bytes = batch_block_bytes();
+    if (hipMalloc(reinterpret_cast<void**>(&b->dev), b->bytes) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&b->host), b->bytes) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void**>(&b->counters), 2 * sizeof(unsigned)) != hipSuccess ||
+        hipMemset(b->counters, 0, 2 * sizeof(unsigned)) != hipSuccess) {
+        (void)hipGetLastError();
+        batch_destroy(b);
+        set_error("serve: batch buffer allocation failed");
+        return OMB_ENOMEM;
+    }
+    *out = b;
+    return OMB_OK;
+}
+
+void batch_destroy(GroupBatch* b) {
+    if (!b) return;
+    if (b->dev) (void)hipFree(b->dev);
+    if (b->host) (void)hipHostFree(b->host);
+    if (b->counters) (void)hipFree(b->counters);
+    delete b;
+}
+
+bool batch_mode_supported(int mode) {
+    return mode == OMB_MODE_PLAIN_F32 || mode == OMB_MODE_SPLIT_BF16 ||
+           mode == OMB_MODE_SPLIT_BF16_P3;
+}
+
+omb_status batch_launch(GroupBatch* b, const omb_gemm_desc* gemms, int n_gemm,
+                        const omb_gemv_desc* gemvs, int n_gemv, int mode, hipStream_t s) {
+    if (!batch_mode_supported(mode)) {
+        set_error("serve batch: mode " + std::to_string(mode) + " is not batched");
+        return OMB_EUNSUPPORTED;
+    }
+    if (n_gemm < 0 || n_gemv < 0 || n_gemm > kBatchMaxGemms || n_gemv > kBatchMaxGemvs) {
+        set_error("serve batch: too many instances");
+        return OMB_EUNSUPPORTED;
+    }
+    OMB_TRY(validate(gemms, n_gemm, gemvs, n_gemv));
+    DGemm* g = reinterpret_cast<DGemm*>(b->host);
+    const size_t voff = round_up(sizeof(DGemm) * kBatchMaxGemms, 256);
+    DGemv* v = reinterpret_cast<DGemv*>(b->host + voff);
+    const size_t ioff = voff + round_up(sizeof(DGemv) * kBatchMaxGemvs, 256);
+    Item* items = reinterpret_cast<Item*>(b->host + ioff);
+    std::vector<Item> list;
+    std::vector<DGemm> gv(n_gemm);
+    std::vector<DGemv> vv(n_gemv);
+    for (int i = 0; i < n_gemm; ++i) {
+        const omb_gemm_desc& d = gemms[i];
+        gv[i] = DGemm{d.M, d.N, d.K, d.A, d.lda, d.B, d.ldb, d.C, d.ldc, d.alpha, d.beta,
+                      nullptr, nullptr};
+        if (d.M == 0 || d.N == 0) continue;
+        int c = choose_class(mode, d);
+        if (c == C128x64) c = C64;  // same bits; keeps the lite kernel
+        const int64_t tmn = ceil_div(d.M, kTileM[c]), tnn = ceil_div(d.N, kTileN[c]);
+        if ((int64_t)list.size() + tmn * tnn > kBatchMaxItems) {
+            set_error("serve batch: too many tiles");
+            return OMB_EUNSUPPORTED;
+        }
+        for (int64_t a = 0; a < tmn; ++a)
+            for (int64_t bb = 0; bb < tnn; ++bb) list.push_back(Item{c, i, (int)a, (int)bb});
+    }
+    for (int i = 0; i < n_gemv; ++i) {
+        const omb_gemv_desc& d = gemvs[i];
+        vv[i] = DGemv{d.M, d.N, d.A, d.lda, d.x, d.y, d.alpha, d.beta};
+        const int64_t chunks = ceil_div(d.M, 64);
+        if ((int64_t)list.size() + chunks > kBatchMaxItems) {
+            set_error("serve batch: too many GEMV chunks");
+            return OMB_EUNSUPPORTED;
+        }
+        for (int64_t c = 0; c < chunks; ++c) list.push_back(Item{CGEMV, i, (int)c, 0});
+    }
+    const int nitems = (int)list.size();
+    if (nitems == 0) return OMB_OK;
+    std::vector<double> cost(list.size());
+    for (size_t i = 0; i < list.size(); ++i) cost[i] = item_cost(list[i], gv, vv);
+    std::vector<size_t> order(list.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t x, size_t y) { return cost[x] > cost[y]; });
+    if (n_gemm) std::memcpy(g, gv.data(), sizeof(DGemm) * n_gemm);
+    if (n_gemv) std::memcpy(v, vv.data(), sizeof(DGemv) * n_gemv);
+    for (size_t i = 0; i < order.size(); ++i) items[i] = list[order[i]];
+    // One copy of the used block (the unused tails of the descriptor arrays
+    // travel too; the block is at most a few hundred KB).
+    const size_t used = ioff + sizeof(Item) * (size_t)nitems;
+    OMB_HIP_TRY(hipMemcpyAsync(b->dev, b->host, used, hipMemcpyHostToDevice, s));
+    const int grid = std::max(1, std::min(nitems, device_cu_count() * lite_per_cu()));
+    const int chunk = std::max(1, std::min(4, nitems / (8 * grid)));
+    PArgs a{reinterpret_cast<const Item*>(b->dev + ioff), nitems,
+            reinterpret_cast<const DGemm*>(b->dev), reinterpret_cast<const DGemv*>(b->dev + voff),
+            b->counters, mode == OMB_MODE_SPLIT_BF16_P3 ? 3 : 6, 0, chunk,
+            (int)ceil_div(nitems, chunk)};
+    hipLaunchKernelGGL(grouped_kernel<false>, dim3(grid), dim3(NT), 0, s, a);
+    OMB_HIP_TRY(hipGetLastError());
+    return OMB_OK;
+}
+
+}  // namespace omb

@@ -94,6 +94,35 @@ def test_mix_equals_solo_bitwise(dev, ompblas, mode):
     assert st["total_flop"] == pytest.approx(want_flop, rel=1e-12)
 
 
+@pytest.mark.parametrize("mode", [1, 0, 3])
+def test_batched_and_unbatched_runs_agree(dev, ompblas, mode):
+    """Small requests eligible together run as grouped launches (the stats
+    count them); the same trace with a stream per request gives bitwise the
+    same outputs, and so does each solo call."""
+    trace = small_trace(ompblas, 200, 4.0e5, seed=54)
+    tr = Trace(dev, ompblas, trace, seed=54, mode=mode)
+    res, st = ompblas.serve_run(tr.requests(), slots=4)
+    torch.cuda.synchronize()
+    assert all(r["status"] == 0 for r in res)
+    assert st["batches"] > 0 and st["batched_requests"] >= 2 * st["batches"], st
+    batched = tr.outputs()
+    tr.clear()
+    res2, st2 = ompblas.serve_run(tr.requests(), slots=4, batching=False)
+    torch.cuda.synchronize()
+    assert all(r["status"] == 0 for r in res2)
+    assert st2["batches"] == 0 and st2["batched_requests"] == 0
+    unbatched = tr.outputs()
+    for i in range(len(trace)):
+        assert not torch.isnan(batched[i]).any(), i
+        assert torch.equal(batched[i], unbatched[i]), (i, trace[i])
+        assert torch.equal(batched[i], tr.solo(i)), (i, trace[i])
+    # every request of a batch shares its launch time and slot
+    by_start = {}
+    for r in res:
+        by_start.setdefault((r["start_s"], r["slot"]), []).append(r)
+    assert max(len(v) for v in by_start.values()) >= 2
+
+
 def test_solo_run_through_the_loop_matches(dev, ompblas):
     """A request run alone through the loop equals the same request in the mix."""
     trace = small_trace(ompblas, 40, 1.0e5, seed=14)
@@ -113,14 +142,15 @@ def test_solo_run_through_the_loop_matches(dev, ompblas):
 
 def test_small_budget_makes_requests_wait(dev, ompblas):
     """A budget of one split GEMM's workspace serialises the GEMMs; they
-    wait for memory instead of failing, and results are unchanged."""
+    wait for memory instead of failing, and results are unchanged.  (Without
+    batching: batched split GEMMs split in the tile and hold no workspace.)"""
     trace = [(ompblas.REQ_GEMM, 300 + 7 * i, 200 + 5 * i, 260 + 3 * i, 0.0) for i in range(12)]
     trace += [(ompblas.REQ_DOT, 0, 5000 + i, 0, 0.0) for i in range(4)]
     tr = Trace(dev, ompblas, trace, seed=24, mode=1)
     need = max(ompblas.serve_request_bytes(r) for r in tr.requests()[:12])
     assert need >= max(ompblas.lib().omb_split_pack_bytes(m, k) +
                        ompblas.lib().omb_split_pack_bytes(n, k) for _, m, n, k, _ in trace[:12])
-    res, st = ompblas.serve_run(tr.requests(), slots=8, hbm_budget=need + 4096)
+    res, st = ompblas.serve_run(tr.requests(), slots=8, hbm_budget=need + 4096, batching=False)
     torch.cuda.synchronize()
     assert all(r["status"] == 0 for r in res)
     assert st["admitted_late"] > 0
@@ -147,11 +177,12 @@ def test_split_k_requests_equal_solo(dev, ompblas, mode):
 
 
 def test_oversize_request_is_refused(dev, ompblas):
-    """A request whose workspace exceeds the budget gets EBUSY; the rest run."""
+    """A request whose workspace exceeds the budget gets EBUSY; the rest run
+    (without batching, where the split GEMM would need no workspace)."""
     trace = [(ompblas.REQ_GEMM, 512, 512, 512, 0.0), (ompblas.REQ_GEMV, 300, 400, 0, 0.0),
              (ompblas.REQ_DOT, 0, 1000, 0, 0.0)]
     tr = Trace(dev, ompblas, trace, seed=34, mode=1)
-    res, st = ompblas.serve_run(tr.requests(), slots=2, hbm_budget=64 * 1024)
+    res, st = ompblas.serve_run(tr.requests(), slots=2, hbm_budget=64 * 1024, batching=False)
     torch.cuda.synchronize()
     assert res[0]["status"] == ompblas.EBUSY
     assert res[0]["ws_bytes"] > 64 * 1024
