@@ -114,33 +114,6 @@ __global__ __launch_bounds__(NT) void copy_lines_kernel(int64_t R, int64_t Cc,
 constexpr int TT = 64;       // transpose tile edge
 constexpr int TPAD = TT + 1;  // LDS row pitch (conflict-free column reads)
 
-// Read a TT x TT tile of S (row-major, lds) at (r0, c0) into LDS: thread t
-// takes the 4-float group t % 16 of rows t / 16 + 16k.
-__device__ inline void tile_load(float (*tile)[TPAD], const float* S, int64_t lds, int64_t R,
-                                 int64_t Cc, int64_t r0, int64_t c0) {
-    const int g = threadIdx.x & 15, rr = threadIdx.x >> 4;
-#pragma unroll
-    for (int k = 0; k < TT / 16; ++k) {
-        const int lr = rr + 16 * k;
-        const int64_t r = r0 + lr, c = c0 + 4 * g;
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (r < R) {
-            if (c + 3 < Cc) {
-                v = ld4u_nt(S + r * lds + c);
-            } else {
-                const float* p = S + r * lds;
-                if (c < Cc) v.x = p[c];
-                if (c + 1 < Cc) v.y = p[c + 1];
-                if (c + 2 < Cc) v.z = p[c + 2];
-            }
-        }
-        tile[lr][4 * g + 0] = v.x;
-        tile[lr][4 * g + 1] = v.y;
-        tile[lr][4 * g + 2] = v.z;
-        tile[lr][4 * g + 3] = v.w;
-    }
-}
-
 // Write the transpose of an LDS tile (rows of S at r0.., columns c0..) into
 // D (row-major, ldd, Cc x R): D[c][r] = tile[r - r0][c - c0].
 __device__ inline void tile_store_t(const float (*tile)[TPAD], float* D, int64_t ldd, int64_t R,
@@ -164,22 +137,112 @@ __device__ inline void tile_store_t(const float (*tile)[TPAD], float* D, int64_t
     }
 }
 
-// D (Cc x R, ldd) = S^T (S: R x Cc, lds), both row-major; 64x64 tiles in a
-// grid stride over a fixed grid (4 workgroups per CU measured 62% of HBM
-// against 56% for one workgroup per tile).
-__global__ __launch_bounds__(NT) void transpose_tiles_kernel(int64_t R, int64_t Cc,
+// The 4 rows of a TT x TT tile of S (row-major, lds) at (r0, c0) that
+// thread t reads: 4-float group t % 16 of rows t / 16 + 16k, zero past the
+// edges; kept in registers so that several tiles' loads are in flight.
+__device__ inline void tile_regs(float4 (&v)[TT / 16], const float* S, int64_t lds, int64_t R,
+                                 int64_t Cc, int64_t r0, int64_t c0) {
+    const int g = threadIdx.x & 15, rr = threadIdx.x >> 4;
+#pragma unroll
+    for (int k = 0; k < TT / 16; ++k) {
+        const int64_t r = r0 + rr + 16 * k, c = c0 + 4 * g;
+        v[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (r < R) {
+            if (c + 3 < Cc) {
+                v[k] = ld4u_nt(S + r * lds + c);
+            } else {
+                const float* p = S + r * lds;
+                if (c < Cc) v[k].x = p[c];
+                if (c + 1 < Cc) v[k].y = p[c + 1];
+                if (c + 2 < Cc) v[k].z = p[c + 2];
+            }
+        }
+    }
+}
+
+__device__ inline void tile_put(float (*tile)[TPAD], const float4 (&v)[TT / 16]) {
+    const int g = threadIdx.x & 15, rr = threadIdx.x >> 4;
+#pragma unroll
+    for (int k = 0; k < TT / 16; ++k) {
+        const int lr = rr + 16 * k;
+        tile[lr][4 * g + 0] = v[k].x;
+        tile[lr][4 * g + 1] = v[k].y;
+        tile[lr][4 * g + 2] = v[k].z;
+        tile[lr][4 * g + 3] = v[k].w;
+    }
+}
+
+// D (Cc x R, ldd) = S^T (S: R x Cc, lds), both row-major, over the whole
+// 64x64 tiles (the first TR x TC tiles): each step of a workgroup reads TPB
+// consecutive tiles into registers (all loads in flight at once), parks
+// them in LDS and writes their transposes; grid stride over a fixed grid.
+// No edge guards: guarded loads and stores in this loop measured slower
+// (0.55 of HBM peak at 16384^2 against 0.65), so partial tiles go to
+// transpose_edge_kernel.  One tile per step measured 0.57 (a plain copy,
+// read and write, reaches 0.75 there).
+constexpr int TPB = 4;
+__global__ __launch_bounds__(NT) void transpose_tiles_kernel(int64_t TR, int64_t TC,
                                                              const float* __restrict__ S,
                                                              int64_t lds, float* __restrict__ D,
                                                              int64_t ldd) {
-    __shared__ float tile[TT][TPAD];
-    const int64_t tc = (Cc + TT - 1) / TT, ntiles = tc * ((R + TT - 1) / TT);
-    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const int64_t c0 = (t % tc) * TT, r0 = (t / tc) * TT;
-        __syncthreads();  // the previous tile has been written out
-        tile_load(tile, S, lds, R, Cc, r0, c0);
+    __shared__ float tile[TPB][TT][TPAD];
+    const int64_t ntiles = TR * TC;
+    const int g = threadIdx.x & 15, rr = threadIdx.x >> 4;
+    for (int64_t t0 = (int64_t)blockIdx.x * TPB; t0 < ntiles; t0 += (int64_t)gridDim.x * TPB) {
+        float4 v[TPB][TT / 16];
+#pragma unroll
+        for (int b = 0; b < TPB; ++b) {
+            const int64_t t = t0 + b < ntiles ? t0 + b : ntiles - 1;
+            const int64_t r0 = (t / TC) * TT, c0 = (t % TC) * TT;
+#pragma unroll
+            for (int k = 0; k < TT / 16; ++k)
+                v[b][k] = ld4u_nt(S + (r0 + rr + 16 * k) * lds + c0 + 4 * g);
+        }
+        __syncthreads();  // the previous step's tiles have been written out
+#pragma unroll
+        for (int b = 0; b < TPB; ++b) tile_put(tile[b], v[b]);
         __syncthreads();
-        tile_store_t(tile, D, ldd, R, Cc, r0, c0);
+#pragma unroll
+        for (int b = 0; b < TPB; ++b) {
+            const int64_t t = t0 + b;
+            if (t >= ntiles) break;
+            const int64_t r0 = (t / TC) * TT, c0 = (t % TC) * TT;
+#pragma unroll
+            for (int k = 0; k < TT / 16; ++k) {
+                const int lc = rr + 16 * k;
+                st4u(D + (c0 + lc) * ldd + r0 + 4 * g,
+                     make_float4(tile[b][4 * g][lc], tile[b][4 * g + 1][lc], tile[b][4 * g + 2][lc],
+                                 tile[b][4 * g + 3][lc]));
+            }
+        }
     }
+}
+
+// The partial tiles of the same transpose: the last tile column (when Cc %
+// TT != 0) for the TR whole tile rows, then the last tile row (when R % TT
+// != 0) across all tile columns; one tile per workgroup, guarded.
+__global__ __launch_bounds__(NT) void transpose_edge_kernel(int64_t R, int64_t Cc,
+                                                            const float* __restrict__ S,
+                                                            int64_t lds, float* __restrict__ D,
+                                                            int64_t ldd) {
+    __shared__ float tile[TT][TPAD];
+    const int64_t TR = R / TT, TC = Cc / TT, tcol = (Cc + TT - 1) / TT;
+    const int64_t ncol = Cc % TT ? TR : 0;  // right strip
+    const int64_t e = blockIdx.x;
+    int64_t tr, tc;
+    if (e < ncol) {
+        tr = e;
+        tc = TC;
+    } else {
+        tr = TR;
+        tc = e - ncol;
+        if (tc >= tcol) return;
+    }
+    float4 v[TT / 16];
+    tile_regs(v, S, lds, R, Cc, tr * TT, tc * TT);
+    tile_put(tile, v);
+    __syncthreads();
+    tile_store_t(tile, D, ldd, R, Cc, tr * TT, tc * TT);
 }
 
 // Square in-place transpose of A (n x n, row-major view with ld): workgroup
@@ -192,8 +255,11 @@ __global__ __launch_bounds__(NT) void transpose_inplace_kernel(int64_t n, float*
     const int64_t bj = blockIdx.x, bi = blockIdx.y;
     if (bi > bj) return;
     const int64_t r0 = bi * TT, c0 = bj * TT;
-    tile_load(t0, A, ld, n, n, r0, c0);
-    if (bi != bj) tile_load(t1, A, ld, n, n, c0, r0);
+    float4 v0[TT / 16], v1[TT / 16];  // both tiles' loads in flight at once
+    tile_regs(v0, A, ld, n, n, r0, c0);
+    if (bi != bj) tile_regs(v1, A, ld, n, n, c0, r0);
+    tile_put(t0, v0);
+    if (bi != bj) tile_put(t1, v1);
     __syncthreads();
     tile_store_t(t0, A, ld, n, n, r0, c0);  // (bi, bj)^T -> (bj, bi)
     if (bi != bj) tile_store_t(t1, A, ld, n, n, c0, r0);
@@ -271,10 +337,21 @@ extern "C" omb_status omb_stranspose(int64_t M, int64_t N, const float* A, int64
     const int64_t R = a_row ? M : N, Cc = a_row ? N : M;
     hipStream_t s = (hipStream_t)stream;
     if (a_row == c_row) {
-        const int64_t ntiles = ceil_div(Cc, TT) * ceil_div(R, TT);
-        const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(ntiles, 4 * device_cu_count()));
-        hipLaunchKernelGGL(transpose_tiles_kernel, dim3((unsigned)grid), dim3(NT), 0, s, R, Cc, A,
-                           lda, C, ldc);
+        const int64_t TR = R / TT, TC = Cc / TT, ntiles = TR * TC;
+        if (ntiles > 0) {
+            const int64_t grid = std::max<int64_t>(
+                1, std::min<int64_t>(ceil_div(ntiles, TPB), 2 * device_cu_count()));
+            hipLaunchKernelGGL(transpose_tiles_kernel, dim3((unsigned)grid), dim3(NT), 0, s, TR,
+                               TC, A, lda, C, ldc);
+        }
+        const int64_t nedge = (Cc % TT ? TR : 0) + (R % TT ? ceil_div(Cc, TT) : 0);
+        if (nedge > 0x7fffffffLL) {
+            set_error("omb_stranspose: too many edge tiles for one launch");
+            return OMB_EUNSUPPORTED;
+        }
+        if (nedge > 0)
+            hipLaunchKernelGGL(transpose_edge_kernel, dim3((unsigned)nedge), dim3(NT), 0, s, R, Cc,
+                               A, lda, C, ldc);
     } else {
         hipLaunchKernelGGL(copy_lines_kernel, dim3((unsigned)row_grid(R)), dim3(NT), 0, s, R, Cc,
                            A, lda, C, ldc);
