@@ -28,13 +28,17 @@
 #ifndef OMPBLAS_DROPIN_HPP
 #define OMPBLAS_DROPIN_HPP
 
+#include <algorithm>
 #include <array>
 #include <cstddef>
 #include <cstdint>
+#include <mutex>
 #include <optional>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <utility>
+#include <vector>
 
 #include "ompblas.h"
 
@@ -79,9 +83,90 @@ inline std::size_t span(int64_t rows, int64_t cols, int64_t rs, int64_t cs) {
     return static_cast<std::size_t>((rows - 1) * rs + (cols - 1) * cs + 1);
 }
 
+// Device buffers for staging host operands, kept between calls: a request
+// takes the smallest free cached block that holds it (blocks are sized in
+// 2 MiB steps), else allocates one; a block comes back after its call has
+// synchronised.  At most `keep_bytes` of free blocks stay cached (larger
+// ones are freed on return); release_staging() frees every free block.
+// The pool is never destroyed (device memory is not freed during static
+// destruction; the process's end reclaims it).
+class staging_pool {
+   public:
+    using block = std::pair<void*, std::size_t>;  // device pointer, bytes
+    static staging_pool& get() {
+        static staging_pool* p = new staging_pool();
+        return *p;
+    }
+    void* acquire(std::size_t bytes) {
+        const std::size_t want = round(bytes);
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            auto best = free_.end();
+            for (auto it = free_.begin(); it != free_.end(); ++it)
+                if (it->second >= want && (best == free_.end() || it->second < best->second))
+                    best = it;
+            if (best != free_.end()) {
+                const block b = *best;
+                free_.erase(best);
+                cached_ -= b.second;
+                used_.push_back(b);
+                return b.first;
+            }
+        }
+        void* d = nullptr;
+        omb_status st = omb_device_alloc(&d, want);
+        if (st != OMB_OK) {  // out of memory: drop the cache and try once more
+            release_staging();
+            st = omb_device_alloc(&d, want);
+        }
+        if (st != OMB_OK)
+            throw std::runtime_error(std::string("amt drop-in staging: ") + omb_last_error());
+        std::lock_guard<std::mutex> lock(mu_);
+        used_.emplace_back(d, want);
+        return d;
+    }
+    void release(void* p) {
+        std::lock_guard<std::mutex> lock(mu_);
+        auto it = std::find_if(used_.begin(), used_.end(),
+                               [&](const block& b) { return b.first == p; });
+        if (it == used_.end()) return;
+        const block b = *it;
+        used_.erase(it);
+        if (cached_ + b.second > keep_bytes) {
+            (void)omb_device_free(b.first);
+            return;
+        }
+        free_.push_back(b);
+        cached_ += b.second;
+    }
+    void release_staging() {
+        std::lock_guard<std::mutex> lock(mu_);
+        for (const block& b : free_) (void)omb_device_free(b.first);
+        free_.clear();
+        cached_ = 0;
+    }
+    std::size_t cached_bytes() {
+        std::lock_guard<std::mutex> lock(mu_);
+        return cached_;
+    }
+    std::size_t keep_bytes = std::size_t(4) << 30;
+
+   private:
+    static std::size_t round(std::size_t b) {
+        const std::size_t g = std::size_t(2) << 20;
+        return (b + g - 1) / g * g;
+    }
+    std::mutex mu_;
+    std::vector<block> free_, used_;
+    std::size_t cached_ = 0;
+};
+
+// Frees the drop-in layer's cached staging buffers (see staging_pool).
+inline void release_staging() { staging_pool::get().release_staging(); }
+
 // A device view of `count` elements at `p`: device memory passes through,
-// host memory is copied to a device buffer (when `in`) and back by
-// finish() (when `out`).
+// host memory is copied to a staging buffer from the pool (when `in`) and
+// back by finish() (when `out`).
 template <class T>
 class staged {
    public:
@@ -91,8 +176,7 @@ class staged {
             dev_ = p;
             return;
         }
-        void* d = nullptr;
-        check(omb_device_alloc(&d, count * sizeof(T)), "amt drop-in staging");
+        void* d = staging_pool::get().acquire(count * sizeof(T));
         dev_ = static_cast<T*>(d);
         owned_ = true;
         if (in) check(omb_memcpy_h2d(d, host_, count * sizeof(T), stream_), "amt drop-in staging");
@@ -102,7 +186,7 @@ class staged {
     ~staged() {
         if (owned_) {
             (void)omb_stream_synchronize(stream_);
-            (void)omb_device_free(const_cast<std::remove_const_t<T>*>(dev_));
+            staging_pool::get().release(const_cast<std::remove_const_t<T>*>(dev_));
         }
     }
     T* get() const { return dev_; }

@@ -466,6 +466,47 @@ double host_call_seconds(F&& f) {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / 3;
 }
 
+// Host operands are staged through the drop-in layer's buffer pool: the
+// second call of the same shapes reuses the cached blocks (the cache does
+// not grow), results stay exact, release_staging() empties the cache, and
+// keep_bytes = 0 frees every block on return.
+void test_staging_pool() {
+    namespace lay = amt::layout;
+    auto& pool = amt::dropin::staging_pool::get();
+    amt::dropin::release_staging();
+    CHECK(pool.cached_bytes() == 0, "pool not empty after release_staging");
+    const std::size_t m = 67, k = 45, n = 53;
+    std::vector<float> a(m * k), b(k * n), c(m * n), want(m * n, 0.f);
+    for (auto& v : a) v = (float)(std::rand() % 100);
+    for (auto& v : b) v = (float)(std::rand() % 100);
+    for (std::size_t i = 0; i < m; ++i)
+        for (std::size_t j = 0; j < n; ++j) {
+            double t = 0.0;
+            for (std::size_t q = 0; q < k; ++q) t += (double)a[i * k + q] * b[q * n + j];
+            want[i * n + j] = (float)t;
+        }
+    const std::size_t na[2] = {m, k}, wa[2] = {k, 1}, nb[2] = {k, n}, wb[2] = {n, 1},
+                      nc[2] = {m, n}, wc[2] = {n, 1};
+    std::size_t cached[2] = {0, 0};
+    for (int call = 0; call < 2; ++call) {
+        std::fill(c.begin(), c.end(), 0.f);
+        amt::mtm_helper(c.data(), nc, wc, a.data(), na, wa, b.data(), nb, wb, lay::last_order{});
+        CHECK(c == want, "staged mtm call %d: wrong result", call);
+        cached[call] = pool.cached_bytes();
+    }
+    CHECK(cached[0] > 0, "no staging block cached after a host call");
+    CHECK(cached[1] == cached[0], "second call grew the cache (%zu -> %zu)", cached[0], cached[1]);
+    amt::dropin::release_staging();
+    CHECK(pool.cached_bytes() == 0, "pool not empty after release_staging");
+    const std::size_t keep = pool.keep_bytes;
+    pool.keep_bytes = 0;
+    std::fill(c.begin(), c.end(), 0.f);
+    amt::mtm_helper(c.data(), nc, wc, a.data(), na, wa, b.data(), nb, wb, lay::last_order{});
+    CHECK(c == want, "staged mtm with keep_bytes = 0: wrong result");
+    CHECK(pool.cached_bytes() == 0, "keep_bytes = 0 still cached %zu bytes", pool.cached_bytes());
+    pool.keep_bytes = keep;
+}
+
 int time_host() {
     namespace lay = amt::layout;
     const std::size_t n = 8192;
@@ -536,6 +577,8 @@ int main(int argc, char** argv) {
         test_tensor_level();
         std::printf("tensor-level closures: %d checks, %d failures\n", g_checks, g_failures);
         test_errors();
+        test_staging_pool();
+        std::printf("staging pool: %d checks, %d failures\n", g_checks, g_failures);
         amt::dropin::settings().gemm_mode = OMB_MODE_SPLIT_BF16;
         for (std::size_t sz : {5, 31}) mtm_case<amt::layout::last_order>(false, true, false, sz, false, "LFL split");
         amt::dropin::settings().gemm_mode = OMB_MODE_SPLIT_FP16;
