@@ -99,6 +99,11 @@ omb_status launch_sgemm_split(int64_t M, int64_t N, int64_t K, float alpha,
 int64_t split_packed_elems(int64_t R, int64_t K);
 omb_status launch_split_pack(int64_t R, int64_t K, const float* X, int64_t rs, int64_t cs,
                              __bf16* out, hipStream_t s, int planes = 3);
+// Both operands of a product, A (M x K) and B^T (N x K, as strided views), in
+// one launch; the same packed form as two launch_split_pack calls.
+omb_status launch_split_pack_pair(int64_t M, int64_t N, int64_t K, const float* A, int64_t rsA,
+                                  int64_t csA, __bf16* ap, const float* Bt, int64_t rsB,
+                                  int64_t csB, __bf16* bp, hipStream_t s, int planes = 3);
 omb_status launch_sgemm_packed(int64_t M, int64_t N, int64_t K, float alpha, const __bf16* ap,
                                const __bf16* bp, float beta, float* C, int64_t ldc, int nprod,
                                hipStream_t s, float* part = nullptr, const int* ea = nullptr,

@@ -145,8 +145,9 @@ omb_status launch_request(const omb_request& r, char* ws, hipStream_t s) {
             const int planes = split_planes(r);
             __bf16* ap = reinterpret_cast<__bf16*>(ws);
             __bf16* bp = ap + packed_elems(r, r.m);
-            OMB_TRY(launch_split_pack(r.m, r.k, r.a, r.k, 1, ap, s, planes));
-            OMB_TRY(launch_split_pack(r.n, r.k, r.b, 1, r.n, bp, s, planes));  // B^T of row-major B
+            // A, and B^T of row-major B, in one launch
+            OMB_TRY(launch_split_pack_pair(r.m, r.n, r.k, r.a, r.k, 1, ap, r.b, 1, r.n, bp, s,
+                                           planes));
             return launch_sgemm_packed(r.m, r.n, r.k, 1.f, ap, bp, 0.f, r.c, r.n,
                                        planes == 2 ? 3 : 6, s,
                                        reinterpret_cast<float*>(ws + packed_bytes(r)), nullptr,
@@ -182,6 +183,7 @@ struct Slot {
 constexpr double kBatchGemmFlop = 1e8;
 constexpr double kBatchGemvBytes = 4.0 * (1 << 20);
 constexpr int kBatchMax = 32;
+constexpr int64_t kBatchMaxItems = 65536;  // the descriptor block's item capacity
 
 bool batchable(const omb_request& r) {
     switch (r.kind) {
@@ -265,6 +267,23 @@ extern "C" omb_status omb_serve_run(omb_request* reqs, int32_t n, const omb_serv
             return OMB_EDEVICE;
         }
     }
+    // Each slot's descriptor block for grouped launches, made before the
+    // clock starts (a pinned and a device allocation take milliseconds); if
+    // they cannot be had, requests run a stream each.
+    bool batching = cfg->no_batching == 0 && n > 1;
+    if (batching) {
+        for (auto& s : slots) {
+            if (batch_create(&s.gb) != OMB_OK) {
+                clear_error();
+                for (auto& t : slots) {
+                    batch_destroy(t.gb);
+                    t.gb = nullptr;
+                }
+                batching = false;
+                break;
+            }
+        }
+    }
     // Operands may still be in production on other streams (the slots do not
     // wait for the null stream): finish all queued device work first.
     if (hipDeviceSynchronize() != hipSuccess) {
@@ -280,7 +299,6 @@ extern "C" omb_status omb_serve_run(omb_request* reqs, int32_t n, const omb_serv
     std::vector<char> deferred(n, 0);  // skipped by an admission scan (slot or memory)
     int next = 0, finished = 0, in_flight = 0, max_in_flight = 0, late = 0;
     int busy_slots = 0, batches = 0, batched = 0;
-    const bool batching = cfg->no_batching == 0;
     std::vector<omb_gemm_desc> bg;
     std::vector<omb_gemv_desc> bv;
     std::vector<int> cand;
@@ -326,6 +344,7 @@ extern "C" omb_status omb_serve_run(omb_request* reqs, int32_t n, const omb_serv
             waiting.size() >= slots.size()) {
             cand.clear();
             int bmode = -1;
+            int64_t items = 0;
             for (int r : waiting) {
                 if ((int)cand.size() >= kBatchMax) break;
                 const omb_request& q = reqs[r];
@@ -334,6 +353,11 @@ extern "C" omb_status omb_serve_run(omb_request* reqs, int32_t n, const omb_serv
                     if (bmode < 0) bmode = q.mode;
                     if (q.mode != bmode) continue;
                 }
+                // at most this many work items (32 x 32 tiles are the smallest)
+                const int64_t it = q.kind == OMB_REQ_GEMM ? ceil_div(q.m, 32) * ceil_div(q.n, 32)
+                                                          : ceil_div(q.m, 64);
+                if (items + it > kBatchMaxItems) continue;
+                items += it;
                 cand.push_back(r);
             }
             if (cand.size() >= 2) {
@@ -341,7 +365,6 @@ extern "C" omb_status omb_serve_run(omb_request* reqs, int32_t n, const omb_serv
                 while (slots[si].busy()) ++si;
                 Slot& s = slots[si];
                 omb_status st = OMB_OK;
-                if (!s.gb) st = batch_create(&s.gb);
                 bg.clear();
                 bv.clear();
                 for (int r : cand) {

@@ -85,18 +85,18 @@ __device__ inline float ld(const float* X, int64_t R, int64_t K, int64_t rs, int
 // F16: rows are scaled by 2^e (e from the row's largest magnitude, `amax`
 // as float bits) and split into two FP16 pieces; the workgroups of the
 // first k group also write each row's e to `exps`.
+// Row tile rt, k group kg (slices kg*SPLIT_SL ..) of one operand.
 template <bool F16, int PLN>
-__global__ __launch_bounds__(NT) void split_pack_kernel(const float* __restrict__ X, int64_t R,
-                                                        int64_t K, int64_t rs, int64_t cs,
-                                                        int64_t nkt, int k_major,
-                                                        __bf16* __restrict__ out,
-                                                        const unsigned* __restrict__ amax,
-                                                        int* __restrict__ exps) {
+__device__ __forceinline__ void pack_tile(const float* __restrict__ X, int64_t R, int64_t K,
+                                          int64_t rs, int64_t cs, int64_t nkt, int k_major,
+                                          __bf16* __restrict__ out,
+                                          const unsigned* __restrict__ amax,
+                                          int* __restrict__ exps, int64_t rt, int64_t kg) {
     __shared__ __attribute__((aligned(16))) __bf16 lds[SPLIT_SL * 3 * TM * TK];  // 24 KB
     const int tid = threadIdx.x;
-    const int64_t rt = blockIdx.x, kt0 = (int64_t)blockIdx.y * SPLIT_SL;
+    const int64_t kt0 = kg * SPLIT_SL;
     const int64_t r0 = rt * TM, k0 = kt0 * TK;
-    if (F16 && blockIdx.y == 0 && tid < TM && r0 + tid < R)
+    if (F16 && kg == 0 && tid < TM && r0 + tid < R)
         exps[r0 + tid] = row_exp<true>(amax, r0 + tid, R);
     // 128-bit loads (any alignment) need a unit stride along the loaded
     // index; other strides take the element path.  The values are the same.
@@ -152,6 +152,35 @@ __global__ __launch_bounds__(NT) void split_pack_kernel(const float* __restrict_
     uint4* dst = reinterpret_cast<uint4*>(out + (rt * nkt + kt0) * (int64_t)slice);
     const int n16 = nsl * slice / 8;
     for (int i = tid; i < n16; i += NT) dst[i] = src[i];
+}
+
+template <bool F16, int PLN>
+__global__ __launch_bounds__(NT) void split_pack_kernel(const float* __restrict__ X, int64_t R,
+                                                        int64_t K, int64_t rs, int64_t cs,
+                                                        int64_t nkt, int k_major,
+                                                        __bf16* __restrict__ out,
+                                                        const unsigned* __restrict__ amax,
+                                                        int* __restrict__ exps) {
+    pack_tile<F16, PLN>(X, R, K, rs, cs, nkt, k_major, out, amax, exps, blockIdx.x, blockIdx.y);
+}
+
+// Both operands of one product in one launch (same K): row tiles of the
+// first, then of the second, along blockIdx.x.
+struct PackOperand {
+    const float* X;
+    int64_t R, rs, cs;
+    int k_major;
+    __bf16* out;
+    int64_t rtiles;
+};
+template <int PLN>
+__global__ __launch_bounds__(NT) void split_pack_pair_kernel(PackOperand a, PackOperand b,
+                                                             int64_t K, int64_t nkt) {
+    const bool first = (int64_t)blockIdx.x < a.rtiles;
+    const PackOperand& o = first ? a : b;
+    const int64_t rt = first ? (int64_t)blockIdx.x : (int64_t)blockIdx.x - a.rtiles;
+    pack_tile<false, PLN>(o.X, o.R, K, o.rs, o.cs, nkt, o.k_major, o.out, nullptr, nullptr, rt,
+                          blockIdx.y);
 }
 
 // ----------------------------------------------------------------- pass 2 --
@@ -390,6 +419,26 @@ omb_status launch_split_pack(int64_t R, int64_t K, const float* X, int64_t rs, i
     return OMB_OK;
 }
 
+omb_status launch_split_pack_pair(int64_t M, int64_t N, int64_t K, const float* A, int64_t rsA,
+                                  int64_t csA, __bf16* ap, const float* Bt, int64_t rsB,
+                                  int64_t csB, __bf16* bp, hipStream_t s, int planes) {
+    if (K <= 0 || (M <= 0 && N <= 0)) return OMB_OK;
+    const int64_t nkt = ceil_div(K, TK), kgroups = ceil_div(nkt, SPLIT_SL);
+    const PackOperand a{A, M, rsA, csA, csA == 1 ? 1 : 0, ap, M > 0 ? ceil_div(M, TM) : 0};
+    const PackOperand b{Bt, N, rsB, csB, csB == 1 ? 1 : 0, bp, N > 0 ? ceil_div(N, TM) : 0};
+    if (a.rtiles + b.rtiles > 0x7fffffffLL || kgroups > 65535) {
+        set_error("split pack: operands too large for one launch");
+        return OMB_EUNSUPPORTED;
+    }
+    const dim3 grid((unsigned)(a.rtiles + b.rtiles), (unsigned)kgroups);
+    if (planes == 2)
+        hipLaunchKernelGGL(split_pack_pair_kernel<2>, grid, dim3(NT), 0, s, a, b, K, nkt);
+    else
+        hipLaunchKernelGGL(split_pack_pair_kernel<3>, grid, dim3(NT), 0, s, a, b, K, nkt);
+    OMB_HIP_TRY(hipGetLastError());
+    return OMB_OK;
+}
+
 omb_status launch_split16_pack(int64_t R, int64_t K, const float* X, int64_t rs, int64_t cs,
                                __bf16* out, unsigned* amax, int* exps, hipStream_t s) {
     if (R <= 0 || K <= 0) return OMB_OK;
@@ -492,8 +541,7 @@ omb_status launch_sgemm_split(int64_t M, int64_t N, int64_t K, float alpha,
     __bf16* bp = ap + a_elems;
     float* part = partial ? reinterpret_cast<float*>(static_cast<char*>(ws->buf) + packed) : nullptr;
     // A as rows m, columns k; B^T as rows n, columns k (element (n,k) = B(k,n)).
-    OMB_TRY(launch_split_pack(M, K, A, rsA, csA, ap, s, planes));
-    OMB_TRY(launch_split_pack(N, K, B, csB, rsB, bp, s, planes));
+    OMB_TRY(launch_split_pack_pair(M, N, K, A, rsA, csA, ap, B, csB, rsB, bp, s, planes));
     return launch_sgemm_packed(M, N, K, alpha, ap, bp, beta, C, ldc, nprod, s, part, nullptr,
                                nullptr, planes);
 }
