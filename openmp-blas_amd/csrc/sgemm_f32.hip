@@ -311,29 +311,37 @@ __global__ __launch_bounds__(NT, occ_of(BM * BN)) void sgemm_f32_kernel(Args q) 
 }
 
 // C = alpha * (sum of the splits' partials, in split order) + beta * C.
+// Rows over blockIdx.y, columns over threads: VEC takes four adjacent
+// columns per thread with 128-bit loads and stores (N, ldc multiples of 4,
+// C 16-byte aligned), else one column.  Eight partials are in flight per
+// step (a long split list is otherwise one memory round trip per partial);
+// every element's adds stay in split order, so VEC changes no bits.
+template <bool VEC>
 __global__ __launch_bounds__(256) void splitk_reduce_kernel(int64_t M, int64_t N, int64_t splits,
                                                             float alpha, float beta,
                                                             const float* __restrict__ part,
                                                             float* C, int64_t ldc) {
-    const int64_t total = M * N;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        float s = part[i];
-        int64_t t = 1;
-        // Eight loads in flight per step (a long split list is otherwise
-        // one memory round trip per partial); the adds stay in split order.
-        for (; t + 8 <= splits; t += 8) {
-            float v[8];
+    constexpr int W = VEC ? 4 : 1;
+    typedef float vf __attribute__((ext_vector_type(W)));
+    const int64_t total = M * N, cols = N / W;
+    for (int64_t m = blockIdx.y; m < M; m += gridDim.y) {
+        for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < cols;
+             q += (int64_t)gridDim.x * blockDim.x) {
+            const int64_t i = m * N + q * W;
+            vf s = *reinterpret_cast<const vf*>(part + i);
+            int64_t t = 1;
+            for (; t + 8 <= splits; t += 8) {
+                vf v[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) v[u] = part[(t + u) * total + i];
+                for (int u = 0; u < 8; ++u) v[u] = *reinterpret_cast<const vf*>(part + (t + u) * total + i);
 #pragma unroll
-            for (int u = 0; u < 8; ++u) s += v[u];
+                for (int u = 0; u < 8; ++u) s += v[u];
+            }
+            for (; t < splits; ++t) s += *reinterpret_cast<const vf*>(part + t * total + i);
+            vf* c = reinterpret_cast<vf*>(C + m * ldc + q * W);
+            const vf v = alpha * s;
+            *c = beta != 0.f ? v + beta * *c : v;
         }
-        for (; t < splits; ++t) s += part[t * total + i];
-        const int64_t m = i / N, n = i - m * N;
-        float* c = C + m * ldc + n;
-        const float v = alpha * s;
-        *c = beta != 0.f ? v + beta * *c : v;
     }
 }
 
@@ -480,10 +488,20 @@ int64_t sgemm_f32_splits(int64_t M, int64_t N, int64_t K, int64_t* kchunk) {
 
 omb_status launch_splitk_reduce(int64_t M, int64_t N, int64_t splits, float alpha, float beta,
                                 const float* part, float* C, int64_t ldc, hipStream_t s) {
-    const unsigned blocks =
-        (unsigned)std::min<int64_t>(ceil_div(M * N, 256), 4 * device_cu_count());
-    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(blocks), dim3(256), 0, s, M, N, splits, alpha,
-                       beta, part, C, ldc);
+    if (M <= 0 || N <= 0) return OMB_OK;
+    const bool vec = N % 4 == 0 && ldc % 4 == 0 && (reinterpret_cast<uintptr_t>(C) & 15) == 0 &&
+                     (reinterpret_cast<uintptr_t>(part) & 15) == 0;
+    const int64_t cols = vec ? N / 4 : N;
+    const int64_t bx = std::min<int64_t>(ceil_div(cols, 256), 4 * device_cu_count());
+    const int64_t by = std::min<int64_t>(
+        std::min<int64_t>(M, 65535), std::max<int64_t>(1, 4 * device_cu_count() / bx));
+    const dim3 grid((unsigned)bx, (unsigned)by);
+    if (vec)
+        hipLaunchKernelGGL(splitk_reduce_kernel<true>, grid, dim3(256), 0, s, M, N, splits, alpha,
+                           beta, part, C, ldc);
+    else
+        hipLaunchKernelGGL(splitk_reduce_kernel<false>, grid, dim3(256), 0, s, M, N, splits,
+                           alpha, beta, part, C, ldc);
     OMB_HIP_TRY(hipGetLastError());
     return OMB_OK;
 }
