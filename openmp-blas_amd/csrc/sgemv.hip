@@ -26,7 +26,7 @@ constexpr int NT = 256;
 // workgroup per CU at 16384 rows, measured 86% of HBM peak) for tall
 // matrices, fewer rows per wavefront when that would leave CUs idle.  A
 // row's order of accumulation is the same for every choice.
-template <int ROWS, int UNROLL>
+template <int ROWS, int UNROLL, bool OFF32 = false>
 __global__ __launch_bounds__(NT) void sgemv_row_kernel(int64_t M, int64_t N, float alpha,
                                                        const float* __restrict__ A, int64_t lda,
                                                        const float* __restrict__ x, float beta,
@@ -41,7 +41,7 @@ __global__ __launch_bounds__(NT) void sgemv_row_kernel(int64_t M, int64_t N, flo
          w * ROWS < M; w += waves) {
         const int64_t r0 = w * ROWS;
         const int nr = (int)std::min<int64_t>(ROWS, M - r0);
-        gemv_rows_wave<ROWS, UNROLL, true>(r0, nr, N, alpha, A, lda, x, beta, y, lane);
+        gemv_rows_wave<ROWS, UNROLL, true, OFF32>(r0, nr, N, alpha, A, lda, x, beta, y, lane);
     }
 }
 
@@ -222,7 +222,10 @@ omb_status launch_sgemv_row(int64_t M, int64_t N, float alpha, const float* A, i
     const int64_t blocks = std::min<int64_t>(ceil_div(ceil_div(M, rows), waves_per_wg),
                                              rows == 16 ? cus : 4 * cus);
     const dim3 grid((unsigned)blocks), block(NT);
-    if (rows == 16)
+    if (rows == 16 && N < kGemvOff32Cols)
+        hipLaunchKernelGGL((sgemv_row_kernel<16, 1, true>), grid, block, 0, s, M, N, alpha, A, lda,
+                           x, beta, y);
+    else if (rows == 16)
         hipLaunchKernelGGL((sgemv_row_kernel<16, 1>), grid, block, 0, s, M, N, alpha, A, lda, x,
                            beta, y);
     else if (rows == 4)

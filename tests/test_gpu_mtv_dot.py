@@ -233,6 +233,35 @@ def test_gemv_bits_do_not_depend_on_alignment(dev, ompblas, order):
     assert torch.allclose(ys[0].double(), want, rtol=0, atol=1e-3)
 
 
+def test_tall_row_gemv_bits_equal_the_grouped_item(dev, ompblas):
+    """A row-major GEMV tall enough for 16-row wavefront groups (at least 16
+    rows per wavefront on every CU) forms its addresses as scalar row bases
+    plus 32-bit lane offsets; the grouped kernel's GEMV item runs the same
+    accumulation with 64-bit addresses.  Ragged N (tail lanes), lda not a
+    multiple of 4 and 4-byte offset A and x: bitwise the same y, and within
+    rounding of an FP64 product."""
+    M, N, ld = 16384 + 37, 2051, 2054
+    buf = torch.empty(1 + M * ld, device=dev)
+    ompblas.fill_uniform(buf, 21, 0)
+    a = buf[1:].view(M, ld)[:, :N]
+    xb = torch.empty(1 + N, device=dev)
+    ompblas.fill_uniform(xb, 21, 1)
+    x = xb[1:]
+    helpers.assert_view_in_bounds(a, M, N, a.stride(0), a.stride(1))
+    y_direct = torch.zeros(M, device=dev)
+    ompblas.check(ompblas.sgemv(M, N, 1.5, a.data_ptr(), ld, 1, x.data_ptr(), 0.0,
+                                y_direct.data_ptr()))
+    y_group = torch.zeros(M, device=dev)
+    plan = ompblas.GroupPlan([], [(M, N, a.data_ptr(), ld, x.data_ptr(), y_group.data_ptr(),
+                                   1.5, 0.0)], ompblas.PLAIN_F32)
+    plan.launch()
+    torch.cuda.synchronize()
+    plan.close()
+    assert torch.equal(y_direct, y_group)
+    want = 1.5 * (a.double() @ x.double())
+    assert torch.allclose(y_direct.double(), want, rtol=0, atol=1e-3)
+
+
 def test_dot_bits_do_not_depend_on_alignment(dev, ompblas):
     n = (1 << 20) + 7
     x0 = torch.empty(n, device=dev)
