@@ -102,6 +102,15 @@ constexpr int kFoldK = 128;  // k per inner accumulation block (as sgemm_f32)
 // fragments for k+1 are read while k's FMAs run.  Every element sums k in
 // order and folds the running sum every 128 k, as sgemm_f32_kernel does, so
 // the results are the same bits.
+// threadIdx.x through an empty asm, read at the start of every item: index
+// math derived from it cannot be hoisted out of the persistent kernel's item
+// loop, where it would hold registers for every item type across all items.
+__device__ __forceinline__ int item_tid() {
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
 template <int N>
 struct FragT;
 template <>
@@ -125,7 +134,7 @@ __device__ __forceinline__ float4 load4_guard(P p, int n) {
 
 __device__ __forceinline__ int clamp4i(int64_t r) { return r <= 0 ? 0 : (r >= 4 ? 4 : (int)r); }
 
-template <int BM, int BN, int TMR, int TNR>
+template <int BM, int BN, int TMR, int TNR, int D>
 __device__ __forceinline__ void tile_f32(const DGemm& g, int tm, int tn, float* smem) {
     constexpr int BK = 16;
     constexpr int H = BM / TMR, W = BN / TNR;
@@ -141,15 +150,15 @@ __device__ __forceinline__ void tile_f32(const DGemm& g, int tm, int tn, float* 
     // Two LDS buffers of [BK][PA] A and [BK][PB] B: slice k+1 is stored into
     // one while slice k is read from the other (one barrier per slice).
     constexpr int SZ = BK * (PA + PB);
-    const int tid = threadIdx.x, tx = tid % W, ty = tid / W;
+    const int tid = item_tid(), tx = tid % W, ty = tid / W;
     const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
     const int64_t M = g.M, N = g.N, K = g.K;
     gcfloat* const gA = as_global(g.A);
     gcfloat* const gB = as_global(g.B);
     gfloat* const gC = as_global(g.C);
 
-    float4 ra[LA], rb[LB];
-    auto load = [&](int64_t k0) {
+    float4 ra[D][LA], rb[D][LB];  // D slices of loads in flight, one set each
+    auto load = [&](int set, int64_t k0) {
 #pragma unroll
         for (int i = 0; i < LA; ++i) {
             const int e = tid + i * NT;
@@ -157,7 +166,7 @@ __device__ __forceinline__ void tile_f32(const DGemm& g, int tm, int tn, float* 
                 const int row = e >> 2, kq = (e & 3) * 4;
                 const int64_t m = m0 + row, k = k0 + kq;
                 const int n = m < M ? clamp4i(K - k) : 0;
-                ra[i] = load4_guard(gA + (n ? m * g.lda + k : 0), n);
+                ra[set][i] = load4_guard(gA + (n ? m * g.lda + k : 0), n);
             }
         }
 #pragma unroll
@@ -167,11 +176,11 @@ __device__ __forceinline__ void tile_f32(const DGemm& g, int tm, int tn, float* 
                 const int kk = e / (BN / 4), nq = (e % (BN / 4)) * 4;
                 const int64_t nn = n0 + nq, k = k0 + kk;
                 const int n = k < K ? clamp4i(N - nn) : 0;
-                rb[i] = load4_guard(gB + (n ? k * g.ldb + nn : 0), n);
+                rb[set][i] = load4_guard(gB + (n ? k * g.ldb + nn : 0), n);
             }
         }
     };
-    auto store = [&](int buf) {
+    auto store = [&](int set, int buf) {
         float* As = smem + buf * SZ;
         float* Bs = As + BK * PA;
 #pragma unroll
@@ -179,10 +188,10 @@ __device__ __forceinline__ void tile_f32(const DGemm& g, int tm, int tn, float* 
             const int e = tid + i * NT;
             if (e < NA4) {
                 const int row = e >> 2, kq = (e & 3) * 4;
-                As[(kq + 0) * PA + row] = ra[i].x;
-                As[(kq + 1) * PA + row] = ra[i].y;
-                As[(kq + 2) * PA + row] = ra[i].z;
-                As[(kq + 3) * PA + row] = ra[i].w;
+                As[(kq + 0) * PA + row] = ra[set][i].x;
+                As[(kq + 1) * PA + row] = ra[set][i].y;
+                As[(kq + 2) * PA + row] = ra[set][i].z;
+                As[(kq + 3) * PA + row] = ra[set][i].w;
             }
         }
 #pragma unroll
@@ -190,7 +199,7 @@ __device__ __forceinline__ void tile_f32(const DGemm& g, int tm, int tn, float* 
             const int e = tid + i * NT;
             if (e < NB4) {
                 const int kk = e / (BN / 4), nq = (e % (BN / 4)) * 4;
-                *reinterpret_cast<float4*>(&Bs[kk * PB + nq]) = rb[i];
+                *reinterpret_cast<float4*>(&Bs[kk * PB + nq]) = rb[set][i];
             }
         }
     };
@@ -236,17 +245,28 @@ __device__ __forceinline__ void tile_f32(const DGemm& g, int tm, int tn, float* 
                 }
         }
     };
-    load(0);
+    // Slices 0 .. D-1 are loaded up front; step kt loads slice kt + D into
+    // the set that held slice kt (stored to LDS in step kt - 1), computes
+    // slice kt from LDS buffer kt % 2 and stores slice kt + 1 into the other
+    // buffer, so each load has D - 1 steps of compute to arrive in.
+#pragma unroll
+    for (int s = 0; s < D; ++s)
+        if (s < nk) load(s, s * BK);
     __syncthreads();  // the previous item is done with LDS
-    store(0);
+    store(0, 0);
     __syncthreads();
-    for (int64_t kt = 0; kt < nk; ++kt) {
-        if (kt + 1 < nk) load((kt + 1) * BK);
-        compute((int)(kt & 1), kt);
-        // The other buffer was last read in step kt - 1, which every thread
-        // finished before the barrier that ended it.
-        if (kt + 1 < nk) store((int)((kt + 1) & 1));
-        __syncthreads();
+    for (int64_t kt0 = 0; kt0 < nk; kt0 += D) {
+#pragma unroll
+        for (int s = 0; s < D; ++s) {
+            const int64_t kt = kt0 + s;
+            if (kt >= nk) break;
+            if (kt + D < nk) load(s, (kt + D) * BK);
+            compute((int)(kt & 1), kt);
+            // The other buffer was last read in step kt - 1, which every
+            // thread finished before the barrier that ended it.
+            if (kt + 1 < nk) store((s + 1) % D, (int)((kt + 1) & 1));
+            __syncthreads();
+        }
     }
 #pragma unroll
     for (int i = 0; i < TMR; ++i) {
@@ -301,7 +321,7 @@ template <int NPROD>
 __device__ __forceinline__ void tile_split(const DGemm& g, int tm, int tn, uint4* smem) {
     using namespace split;
     constexpr int P = 64;  // rows per plane for A and for B^T
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = item_tid(), lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int frow = lane & 31, fh = lane >> 5;
     const int64_t m0 = (int64_t)tm * P, n0 = (int64_t)tn * P;
@@ -411,7 +431,7 @@ __global__ __launch_bounds__(NT) void grouped_exps_kernel(const ExpItem* __restr
 __device__ __forceinline__ void tile_split16(const DGemm& g, int tm, int tn, uint4* smem) {
     using namespace split;
     constexpr int P = 64;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = item_tid(), lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int frow = lane & 31, fh = lane >> 5;
     const int64_t m0 = (int64_t)tm * P, n0 = (int64_t)tn * P;
@@ -465,7 +485,8 @@ __device__ __forceinline__ void tile_split16(const DGemm& g, int tm, int tn, uin
 
 // ------------------------------------------------------------- GEMV item --
 __device__ __forceinline__ void gemv_item(const DGemv& v, int chunk) {
-    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int tid = item_tid();
+    const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     // wavefront w takes rows chunk*64 + 16w .. +15 in one pass (16 rows of
     // 128-bit loads in flight per lane)
     const int64_t r0 = (int64_t)chunk * 64 + wave * 16;
@@ -479,6 +500,10 @@ __device__ __forceinline__ void gemv_item(const DGemv& v, int chunk) {
 // LDS: the largest plain class footprint (128x64: two buffers of
 // 16 x (132 + 68) floats); the split tile uses 12 KB.
 constexpr int kSmemFloats = 2 * (16 * (128 + 4) + 16 * (64 + 4)) + 16;
+
+// Slices of global loads in flight for the 32x32 and 64x64 plain tiles (8
+// VGPRs each).
+constexpr int kPrefetch = 4;
 
 // The lite build is bounded to 3 workgroups per CU (at most 168 VGPRs), the
 // big one (128x64 tiles) to 2.
@@ -503,10 +528,14 @@ __global__ __launch_bounds__(NT, BIG ? 2 : 3) void grouped_kernel(PArgs p) {
             w.tm = __builtin_amdgcn_readfirstlane(w.tm);
             w.tn = __builtin_amdgcn_readfirstlane(w.tn);
             switch (w.cls) {
-                case C32: tile_f32<32, 32, 2, 2>(p.gemms[w.inst], w.tm, w.tn, smem); break;
-                case C64: tile_f32<64, 64, 4, 4>(p.gemms[w.inst], w.tm, w.tn, smem); break;
+                case C32:
+                    tile_f32<32, 32, 2, 2, kPrefetch>(p.gemms[w.inst], w.tm, w.tn, smem);
+                    break;
+                case C64:
+                    tile_f32<64, 64, 4, 4, kPrefetch>(p.gemms[w.inst], w.tm, w.tn, smem);
+                    break;
                 case C128x64:
-                    if (BIG) tile_f32<128, 64, 8, 4>(p.gemms[w.inst], w.tm, w.tn, smem);
+                    if (BIG) tile_f32<128, 64, 8, 4, 1>(p.gemms[w.inst], w.tm, w.tn, smem);
                     break;
                 case CSPLIT:
                     if (p.f16)
