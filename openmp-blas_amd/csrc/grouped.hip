@@ -505,10 +505,12 @@ constexpr int kSmemFloats = 2 * (16 * (128 + 4) + 16 * (64 + 4)) + 16;
 // VGPRs each).
 constexpr int kPrefetch = 4;
 
-// The lite build is bounded to 3 workgroups per CU (at most 168 VGPRs), the
-// big one (128x64 tiles) to 2.
+// The lite build is bounded to 4 workgroups per CU (at most 128 VGPRs): the
+// C4 plain plan measured 3.48 ms so, against 3.55-3.63 ms for builds of 129
+// and 165 VGPRs (3 per CU) that spent the registers on deeper split-tile
+// prefetch.  The big build (128x64 tiles) is bounded to 2.
 template <bool BIG>
-__global__ __launch_bounds__(NT, BIG ? 2 : 3) void grouped_kernel(PArgs p) {
+__global__ __launch_bounds__(NT, BIG ? 2 : 4) void grouped_kernel(PArgs p) {
     __shared__ __attribute__((aligned(16))) float smem[kSmemFloats];
     __shared__ int s_chunk;
     // Chunks of consecutive (cost-sorted) items: workgroup b starts with
