@@ -91,17 +91,6 @@ struct PArgs {
 constexpr int NT = 256;
 constexpr int kFoldK = 128;  // k per inner accumulation block (as sgemm_f32)
 
-// ------------------------------------------------------------ plain tile --
-// BM x BN tile of C = alpha*A*B + beta*C (all row-major); 256 threads on an
-// H x W = (BM/TMR) x (BN/TNR) grid.  Thread (ty, tx) owns GR fragments of FR
-// consecutive rows, at rows g*(BM/GR) + ty*FR + i, and GC fragments of FC
-// consecutive columns (FR = min(TMR, 4), GR = TMR / FR; likewise for
-// columns), so every LDS read is one 64- or 128-bit access.  Slices of 16 k
-// are loaded with 128-bit global loads along the unit-stride index, staged
-// through two LDS buffers, and the next slice is prefetched into registers; LDS
-// fragments for k+1 are read while k's FMAs run.  Every element sums k in
-// order and folds the running sum every 128 k, as sgemm_f32_kernel does, so
-// the results are the same bits.
 // threadIdx.x through an empty asm, read at the start of every item: index
 // math derived from it cannot be hoisted out of the persistent kernel's item
 // loop, where it would hold registers for every item type across all items.
@@ -111,6 +100,17 @@ __device__ __forceinline__ int item_tid() {
     return t;
 }
 
+// ------------------------------------------------------------ plain tile --
+// BM x BN tile of C = alpha*A*B + beta*C (all row-major); 256 threads on an
+// H x W = (BM/TMR) x (BN/TNR) grid.  Thread (ty, tx) owns GR fragments of FR
+// consecutive rows, at rows g*(BM/GR) + ty*FR + i, and GC fragments of FC
+// consecutive columns (FR = min(TMR, 4), GR = TMR / FR; likewise for
+// columns), so every LDS read is one 64- or 128-bit access.  Slices of 16 k
+// are loaded with 128-bit global loads along the unit-stride index, staged
+// through two LDS buffers, and the next D slices are prefetched into D
+// register sets; LDS fragments for k+1 are read while k's FMAs run.  Every
+// element sums k in order and folds the running sum every 128 k, as
+// sgemm_f32_kernel does, so the results are the same bits.
 template <int N>
 struct FragT;
 template <>
