@@ -164,6 +164,10 @@ void batch_destroy(GroupBatch* b);
 bool batch_mode_supported(int mode);
 omb_status batch_launch(GroupBatch* b, const omb_gemm_desc* gemms, int n_gemm,
                         const omb_gemv_desc* gemvs, int n_gemv, int mode, hipStream_t s);
+// One launch of the persistent kernel over no items on stream s: the first
+// launch of a kernel in a process costs milliseconds, which a caller timing
+// its batches pays here, before its clock starts.
+omb_status batch_warm(GroupBatch* b, hipStream_t s);
 
 // ----------------------------------------------------- device helpers -----
 #if defined(__HIPCC__)
