@@ -488,12 +488,17 @@ __device__ __forceinline__ void gemv_item(const DGemv& v, int chunk) {
     const int tid = item_tid();
     const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     // wavefront w takes rows chunk*64 + 16w .. +15 in one pass (16 rows of
-    // 128-bit loads in flight per lane)
+    // 128-bit loads in flight per lane); rows under 2^29 columns use scalar
+    // row bases and 32-bit lane offsets (GEMV-only C4 2.63 -> 2.61 ms)
     const int64_t r0 = (int64_t)chunk * 64 + wave * 16;
     if (r0 >= v.M) return;
     const int nr = (int)(v.M - r0 < 16 ? v.M - r0 : 16);
-    gemv_rows_wave<16, 1, false>(r0, nr, v.N, v.alpha, as_global(v.A), v.lda, as_global(v.x), v.beta,
-                          as_global(v.y), lane);
+    if (v.N < kGemvOff32Cols)
+        gemv_rows_wave<16, 1, false, true>(r0, nr, v.N, v.alpha, as_global(v.A), v.lda,
+                                           as_global(v.x), v.beta, as_global(v.y), lane);
+    else
+        gemv_rows_wave<16, 1, false>(r0, nr, v.N, v.alpha, as_global(v.A), v.lda, as_global(v.x), v.beta,
+                                     as_global(v.y), lane);
 }
 
 // ------------------------------------------------------- persistent kernel --

@@ -236,10 +236,11 @@ def test_gemv_bits_do_not_depend_on_alignment(dev, ompblas, order):
 def test_tall_row_gemv_bits_equal_the_grouped_item(dev, ompblas):
     """A row-major GEMV tall enough for 16-row wavefront groups (at least 16
     rows per wavefront on every CU) forms its addresses as scalar row bases
-    plus 32-bit lane offsets; the grouped kernel's GEMV item runs the same
-    accumulation with 64-bit addresses.  Ragged N (tail lanes), lda not a
-    multiple of 4 and 4-byte offset A and x: bitwise the same y, and within
-    rounding of an FP64 product."""
+    plus 32-bit lane offsets, as the grouped kernel's GEMV item does; a
+    short call (4- or 1-row groups) keeps 64-bit addresses.  Ragged N (tail
+    lanes), lda not a multiple of 4 and 4-byte offset A and x: the tall call,
+    the grouped item and the short call on the first rows give bitwise the
+    same y, within rounding of an FP64 product."""
     M, N, ld = 16384 + 37, 2051, 2054
     buf = torch.empty(1 + M * ld, device=dev)
     ompblas.fill_uniform(buf, 21, 0)
@@ -258,6 +259,11 @@ def test_tall_row_gemv_bits_equal_the_grouped_item(dev, ompblas):
     torch.cuda.synchronize()
     plan.close()
     assert torch.equal(y_direct, y_group)
+    for rows in (5000, 97):  # 4-row and 1-row groups on a 256-CU device
+        y_short = torch.zeros(rows, device=dev)
+        ompblas.check(ompblas.sgemv(rows, N, 1.5, a.data_ptr(), ld, 1, x.data_ptr(), 0.0,
+                                    y_short.data_ptr()))
+        assert torch.equal(y_short, y_direct[:rows]), rows
     want = 1.5 * (a.double() @ x.double())
     assert torch.allclose(y_direct.double(), want, rtol=0, atol=1e-3)
 
