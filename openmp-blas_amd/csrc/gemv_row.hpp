@@ -35,7 +35,8 @@ constexpr int64_t kGemvOff32Cols = (int64_t)1 << 29;
 // normally -- measured 3.5% faster there.  OFF32, for N < kGemvOff32Cols and
 // a wavefront-uniform r0: the row bases are scalar and every load adds one
 // 32-bit lane offset, which frees about 90 VGPRs of per-row addresses and
-// measured 1.5% more bandwidth in the standalone row kernel.)
+// measured 1.5% more bandwidth in the standalone row kernel, 0.8% in the
+// grouped kernel's GEMV items.)
 template <int ROWS, int UNROLL, bool STREAM, bool OFF32 = false, typename PA, typename PX,
           typename PY>
 __device__ __forceinline__ void gemv_rows_wave(int64_t r0, int nr, int64_t N, float alpha,
