@@ -251,7 +251,7 @@ __device__ __forceinline__ void tile_f32(const DGemm& g, int tm, int tn, float* 
     // buffer, so each load has D - 1 steps of compute to arrive in.
 #pragma unroll
     for (int s = 0; s < D; ++s)
-        if (s < nk) load(s, s * BK);
+        if (s == 0 || s < nk) load(s, s * BK);  // slice 0 even when K = 0 (zeros)
     __syncthreads();  // the previous item is done with LDS
     store(0, 0);
     __syncthreads();
