@@ -272,24 +272,75 @@ __global__ __launch_bounds__(NT, 2) void sgemm_bf16x3_kernel(MArgs p) {
     };
     uint4 x0, x1, x2, x3, x4, x5;  // even slices in flight
     uint4 y0, y1, y2, y3, y4, y5;  // odd slices in flight
-    OMB_GLOAD(0, x0, x1, x2, x3, x4, x5);
-    if (nkt > 1) OMB_GLOAD(1, y0, y1, y2, y3, y4, y5);
-    OMB_LSTORE(lds[0], x0, x1, x2, x3, x4, x5);
-    __syncthreads();
-    for (int64_t kt = 0; kt < nkt; kt += 2) {
-        // even slice kt from LDS 0; y* hold kt+1; load kt+2 into x*
-        if (kt + 2 < nkt) OMB_GLOAD(kt + 2, x0, x1, x2, x3, x4, x5);
-        compute(0, kt);
-        fold_at(kt);
-        if (kt + 1 < nkt) OMB_LSTORE(lds[1], y0, y1, y2, y3, y4, y5);
+    if constexpr (NPROD == 3 && PL == 2) {
+        // Two-plane forms: the fragments are double-buffered in registers.
+        // After each barrier a wave issues the next slice's loads and LDS
+        // reads, then the MFMAs of the slice whose fragments it holds, so
+        // the LDS reads overlap the MFMAs (the scheduling barriers keep that
+        // order).  The loop runs whole slice pairs, with an odd last slice
+        // after it, and loads and stores are unconditional with the slice
+        // index clamped to the last one (past the end a store writes a copy
+        // of the last slice into a buffer that is not read again): with one
+        // path through the loop the compiler's wait counts are exact, and a
+        // store waits only for its own loads.  The MFMA and fold order is
+        // that of the loop below, so the results are the same.
+        V fa0[2][2], fb0[2][2];  // fragments of an even slice
+        V fa1[2][2], fb1[2][2];  // of an odd slice
+        const int64_t klast = nkt - 1;  // >= 0: every split holds a slice
+        auto kc = [&](int64_t k) { return k < klast ? k : klast; };
+        OMB_GLOAD(0, x0, x1, x2, x3, x4, x5);
+        OMB_GLOAD(kc(1), y0, y1, y2, y3, y4, y5);
+        OMB_LSTORE(lds[0], x0, x1, x2, x3, x4, x5);
+        OMB_LSTORE(lds[1], y0, y1, y2, y3, y4, y5);
+        OMB_GLOAD(kc(2), x0, x1, x2, x3, x4, x5);
         __syncthreads();
-        if (kt + 1 >= nkt) break;
-        // odd slice kt+1 from LDS 1; x* hold kt+2; load kt+3 into y*
-        if (kt + 3 < nkt) OMB_GLOAD(kt + 3, y0, y1, y2, y3, y4, y5);
-        compute(1, kt + 1);
-        fold_at(kt + 1);
-        if (kt + 2 < nkt) OMB_LSTORE(lds[0], x0, x1, x2, x3, x4, x5);
+        read_frags<2, 2, 2, TM, TN, V, PL>(lds[0], wm * 64, wn * 64, frow, fh, fa0, fb0);
+        __syncthreads();  // every wave has read slice 0 before slice 2 replaces it
+        int64_t kt = 0;
+        for (; kt + 1 < nkt; kt += 2) {
+            // fa0/fb0: slice kt; LDS 1: slice kt+1; x*: slice kt+2
+            read_frags<2, 2, 2, TM, TN, V, PL>(lds[1], wm * 64, wn * 64, frow, fh, fa1, fb1);
+            OMB_GLOAD(kc(kt + 3), y0, y1, y2, y3, y4, y5);
+            __builtin_amdgcn_sched_barrier(0);
+            mfma_frags<NPROD, 2, 2, 2, V>(fa0, fb0, inner);
+            __builtin_amdgcn_sched_barrier(0);
+            fold_at(kt);
+            OMB_LSTORE(lds[0], x0, x1, x2, x3, x4, x5);
+            __syncthreads();
+            // fa1/fb1: slice kt+1; LDS 0: slice kt+2; y*: slice kt+3
+            read_frags<2, 2, 2, TM, TN, V, PL>(lds[0], wm * 64, wn * 64, frow, fh, fa0, fb0);
+            OMB_GLOAD(kc(kt + 4), x0, x1, x2, x3, x4, x5);
+            __builtin_amdgcn_sched_barrier(0);
+            mfma_frags<NPROD, 2, 2, 2, V>(fa1, fb1, inner);
+            __builtin_amdgcn_sched_barrier(0);
+            fold_at(kt + 1);
+            OMB_LSTORE(lds[1], y0, y1, y2, y3, y4, y5);
+            __syncthreads();
+        }
+        if (kt < nkt) {  // odd slice count: fa0/fb0 hold the last slice
+            mfma_frags<NPROD, 2, 2, 2, V>(fa0, fb0, inner);
+            fold_at(kt);
+        }
+    } else {
+        OMB_GLOAD(0, x0, x1, x2, x3, x4, x5);
+        if (nkt > 1) OMB_GLOAD(1, y0, y1, y2, y3, y4, y5);
+        OMB_LSTORE(lds[0], x0, x1, x2, x3, x4, x5);
         __syncthreads();
+        for (int64_t kt = 0; kt < nkt; kt += 2) {
+            // even slice kt from LDS 0; y* hold kt+1; load kt+2 into x*
+            if (kt + 2 < nkt) OMB_GLOAD(kt + 2, x0, x1, x2, x3, x4, x5);
+            compute(0, kt);
+            fold_at(kt);
+            if (kt + 1 < nkt) OMB_LSTORE(lds[1], y0, y1, y2, y3, y4, y5);
+            __syncthreads();
+            if (kt + 1 >= nkt) break;
+            // odd slice kt+1 from LDS 1; x* hold kt+2; load kt+3 into y*
+            if (kt + 3 < nkt) OMB_GLOAD(kt + 3, y0, y1, y2, y3, y4, y5);
+            compute(1, kt + 1);
+            fold_at(kt + 1);
+            if (kt + 2 < nkt) OMB_LSTORE(lds[0], x0, x1, x2, x3, x4, x5);
+            __syncthreads();
+        }
     }
 
     // (The exponents scale a split's partial sums before they are stored:
@@ -490,6 +541,10 @@ omb_status launch_sgemm_packed(int64_t M, int64_t N, int64_t K, float alpha, con
     if (tiles_m * tiles_n > 0x7fffffffLL) {
         set_error("sgemm split: too many tiles for one launch");
         return OMB_EUNSUPPORTED;
+    }
+    if (K <= 0) {  // the kernel loads at least one slice per split
+        set_error("sgemm split: K must be positive");
+        return OMB_EINVAL;
     }
     const int64_t nkt = ceil_div(K, TK);
     MArgs a{ap, bp, nkt, C, ldc, M, N, alpha, beta, tiles_m, tiles_n, nkt, nullptr, ea, eb};

@@ -85,22 +85,17 @@ __device__ inline void split8(float4 lo, float4 hi, uint4& p0, uint4& p1, uint4&
 namespace omb {
 namespace split {
 
-// MFMAs of one k slice for one wavefront owning BI x BJ blocks of 32x32 of C
-// (rows wrow0.., columns wcol0.. of the slice): fragments from the swizzled
-// LDS slice (PA rows per A plane, PB per B plane), smallest products first,
-// all into the inner accumulators.
+// Fragments of one k slice for one wavefront owning BI x BJ blocks of 32x32
+// of C (rows wrow0.., columns wcol0.. of the slice), from the swizzled LDS
+// slice (PA rows per A plane, PB per B plane; PLA planes in the A part).
 // V is the 8-element operand vector of the MFMA: bf16x8 (BF16 pieces) or
-// f16x8 (scaled FP16 pieces, 3 products).
-// PLA: planes of the A part of the LDS slice (3, or 2 for the FP16 form).
-template <int NPROD, int BI = 2, int BJ = 2, int PA = TM, int PB = TN, typename V = bf16x8,
-          int PLA = 3>
-__device__ __forceinline__ void mfma_slice(const uint4* slice, int wrow0, int wcol0, int frow,
-                                           int fh, f32x16 (&inner)[BI][BJ]) {
-    constexpr int NPLANES = NPROD == 6 ? 3 : 2;
+// f16x8 (scaled FP16 pieces).
+template <int NPLANES, int BI, int BJ, int PA, int PB, typename V, int PLA>
+__device__ __forceinline__ void read_frags(const uint4* slice, int wrow0, int wcol0, int frow,
+                                           int fh, V (&a)[NPLANES][BI], V (&b)[NPLANES][BJ]) {
     static_assert(NPLANES <= PLA, "the slice holds every plane the products use");
     const uint4* la = slice;
     const uint4* lb = slice + PLA * 2 * PA;
-    V a[NPLANES][BI], b[NPLANES][BJ];
 #pragma unroll
     for (int pl = 0; pl < NPLANES; ++pl) {
 #pragma unroll
@@ -114,11 +109,20 @@ __device__ __forceinline__ void mfma_slice(const uint4* slice, int wrow0, int wc
             b[pl][j] = *reinterpret_cast<const V*>(&vb);
         }
     }
+}
+
+// The slice's MFMAs from its fragments, smallest products first, all into
+// the inner accumulators (NPROD = 6: all pieces of 3 planes; 3: the three
+// largest products of 2 planes).
+template <int NPROD, int NPLANES, int BI, int BJ, typename V>
+__device__ __forceinline__ void mfma_frags(const V (&a)[NPLANES][BI], const V (&b)[NPLANES][BJ],
+                                           f32x16 (&inner)[BI][BJ]) {
+    static_assert(NPLANES == (NPROD == 6 ? 3 : 2), "planes of the products");
 #pragma unroll
     for (int i = 0; i < BI; ++i)
 #pragma unroll
         for (int j = 0; j < BJ; ++j) {
-            if (NPROD == 6) {
+            if constexpr (NPROD == 6) {
                 inner[i][j] = mfma(a[0][i], b[2][j], inner[i][j]);
                 inner[i][j] = mfma(a[1][i], b[1][j], inner[i][j]);
                 inner[i][j] = mfma(a[2][i], b[0][j], inner[i][j]);
@@ -127,6 +131,18 @@ __device__ __forceinline__ void mfma_slice(const uint4* slice, int wrow0, int wc
             inner[i][j] = mfma(a[1][i], b[0][j], inner[i][j]);
             inner[i][j] = mfma(a[0][i], b[0][j], inner[i][j]);
         }
+}
+
+// MFMAs of one k slice: its fragment reads, then its MFMAs.
+// PLA: planes of the A part of the LDS slice (3, or 2 for the two-plane forms).
+template <int NPROD, int BI = 2, int BJ = 2, int PA = TM, int PB = TN, typename V = bf16x8,
+          int PLA = 3>
+__device__ __forceinline__ void mfma_slice(const uint4* slice, int wrow0, int wcol0, int frow,
+                                           int fh, f32x16 (&inner)[BI][BJ]) {
+    constexpr int NPLANES = NPROD == 6 ? 3 : 2;
+    V a[NPLANES][BI], b[NPLANES][BJ];
+    read_frags<NPLANES, BI, BJ, PA, PB, V, PLA>(slice, wrow0, wcol0, frow, fh, a, b);
+    mfma_frags<NPROD, NPLANES, BI, BJ, V>(a, b, inner);
 }
 
 // outer += inner; inner = 0 (FP32 adds: the running sum over K is rounded
