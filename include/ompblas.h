@@ -282,6 +282,32 @@ typedef struct {
 
 omb_status omb_serve_run(omb_request* reqs, int32_t n, const omb_serve_config* cfg,
                          omb_request_result* results, omb_serve_stats* stats);
+/* Live submission: the same admission, batching and slots as omb_serve_run,
+ * on a worker thread that takes requests as they are submitted instead of
+ * from a trace (no counterpart in the reference, whose calls are
+ * synchronous; each request runs one of its mtm / mtv / dot calls,
+ * include/mtm.hpp:116-206, mtv.hpp:72-98, dot.hpp:143-174).
+ * omb_server_open makes the slots' streams and a workspace arena of
+ * cfg->slots x max_request_bytes (within the HBM budget) on the current
+ * device; a request needing more is refused with OMB_EBUSY in its result.
+ * omb_server_submit queues a copy of *r (arrival_s is set to the submission
+ * time) and returns its ticket (0, 1, 2, ... in submission order); its
+ * operands must be complete on the device when it is submitted (the slots'
+ * streams do not wait for other streams) and stay valid until its result is
+ * published.  omb_server_poll sets *done (and *result once done);
+ * omb_server_wait blocks until the ticket's result is published.
+ * omb_server_close waits for every submitted request, frees the server and
+ * fills *stats (latencies from submission to observed completion).  Submit,
+ * poll and wait may be called from any threads; close once, after the
+ * last submit. */
+typedef struct omb_server omb_server;
+omb_status omb_server_open(const omb_serve_config* cfg, int64_t max_request_bytes,
+                           omb_server** server);
+omb_status omb_server_submit(omb_server* server, const omb_request* r, int64_t* ticket);
+omb_status omb_server_poll(omb_server* server, int64_t ticket, int32_t* done,
+                           omb_request_result* result);
+omb_status omb_server_wait(omb_server* server, int64_t ticket, omb_request_result* result);
+omb_status omb_server_close(omb_server* server, omb_serve_stats* stats);
 /* The workspace bytes omb_serve_run will hold for request r while it runs
  * (the value it fills into ws_bytes), for sizing hbm_budget; -1 for an
  * invalid request.  Split-BF16 GEMMs hold their packed operands, GEMMs

@@ -140,6 +140,11 @@ _SIGS = {
     "omb_serve_request_bytes": (_I64, [ctypes.POINTER(Request)]),
     "omb_serve_run": (_I32, [ctypes.POINTER(Request), _I32, ctypes.POINTER(ServeConfig),
                              ctypes.POINTER(RequestResult), ctypes.POINTER(ServeStats)]),
+    "omb_server_open": (_I32, [ctypes.POINTER(ServeConfig), _I64, ctypes.POINTER(_VP)]),
+    "omb_server_submit": (_I32, [_VP, ctypes.POINTER(Request), ctypes.POINTER(_I64)]),
+    "omb_server_poll": (_I32, [_VP, _I64, ctypes.POINTER(_I32), ctypes.POINTER(RequestResult)]),
+    "omb_server_wait": (_I32, [_VP, _I64, ctypes.POINTER(RequestResult)]),
+    "omb_server_close": (_I32, [_VP, ctypes.POINTER(ServeStats)]),
     "omb_is_device_pointer": (_I32, [_VP]),
     "omb_device_alloc": (_I32, [ctypes.POINTER(_VP), ctypes.c_size_t]),
     "omb_device_free": (_I32, [_VP]),
@@ -482,6 +487,60 @@ def serve_run(requests, slots: int = 8, hbm_budget: int = 0, hbm_reserve: int = 
                         status=r.status, slot=r.slot, ws_bytes=reqs[i].ws_bytes))
     stats = {name: getattr(st, name) for name, _ in ServeStats._fields_}
     return out, stats
+
+
+def _result_dict(r) -> dict:
+    return dict(start_s=r.start_s, done_s=r.done_s, latency_s=r.latency_s, status=r.status,
+                slot=r.slot)
+
+
+class Server:
+    """Live submission through omb_server_*: requests are admitted as they
+    are submitted (the slots, arena and batching of serve_run) by a worker
+    thread in the library.  submit() returns a ticket; wait(ticket) blocks
+    for its result dict; close() waits for every request and returns the
+    statistics.  Operands must be complete on the device at submission."""
+
+    def __init__(self, slots: int = 8, max_request_bytes: int = 64 << 20, hbm_budget: int = 0,
+                 hbm_reserve: int = 1 << 30, spin_poll: bool = True, batching: bool = True):
+        cfg = ServeConfig(slots, hbm_budget, hbm_reserve, 1 if spin_poll else 0,
+                          0 if batching else 1)
+        self._h = ctypes.c_void_p()
+        check(lib().omb_server_open(ctypes.byref(cfg), max_request_bytes, ctypes.byref(self._h)))
+
+    def submit(self, request) -> int:
+        t = _I64()
+        check(lib().omb_server_submit(self._h, ctypes.byref(request), ctypes.byref(t)))
+        return int(t.value)
+
+    def poll(self, ticket: int) -> Optional[dict]:
+        done, r = _I32(), RequestResult()
+        check(lib().omb_server_poll(self._h, ticket, ctypes.byref(done), ctypes.byref(r)))
+        return _result_dict(r) if done.value else None
+
+    def wait(self, ticket: int) -> dict:
+        r = RequestResult()
+        check(lib().omb_server_wait(self._h, ticket, ctypes.byref(r)))
+        return _result_dict(r)
+
+    def close(self) -> dict:
+        st = ServeStats()
+        if self._h:
+            h, self._h = self._h, ctypes.c_void_p()
+            check(lib().omb_server_close(h, ctypes.byref(st)))
+        return {name: getattr(st, name) for name, _ in ServeStats._fields_}
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def serve_request_bytes(request) -> int:

@@ -152,6 +152,28 @@ def test_serve_run_rejects_bad_arguments(ompblas):
     assert "omb_serve_run" in ompblas.last_error()
 
 
+def test_live_server_rejects_bad_arguments(ompblas):
+    """omb_server_*: configuration and handle validation runs before any
+    device call (no server is opened here)."""
+    lib = ompblas.lib()
+    Cfg = ompblas.ServeConfig
+    h = ctypes.c_void_p()
+    assert lib.omb_server_open(None, 0, ctypes.byref(h)) == ompblas.EINVAL
+    assert lib.omb_server_open(ctypes.byref(Cfg(0, 0, 0, 1)), 0, ctypes.byref(h)) == -1
+    assert lib.omb_server_open(ctypes.byref(Cfg(65, 0, 0, 1)), 0, ctypes.byref(h)) == -1
+    assert lib.omb_server_open(ctypes.byref(Cfg(2, -5, 0, 1)), 0, ctypes.byref(h)) == -1
+    assert lib.omb_server_open(ctypes.byref(Cfg(2, 0, 0, 1)), -1, ctypes.byref(h)) == -1
+    assert lib.omb_server_open(ctypes.byref(Cfg(2, 0, 0, 1)), 0, None) == -1
+    assert "omb_server_open" in ompblas.last_error()
+    r = ompblas.Request(ompblas.REQ_DOT, 0, 0, 8, 0, 0.0, 16, 16, 16, 0)
+    t = ctypes.c_int64()
+    d = ctypes.c_int32()
+    assert lib.omb_server_submit(None, ctypes.byref(r), ctypes.byref(t)) == -1
+    assert lib.omb_server_poll(None, 0, ctypes.byref(d), None) == -1
+    assert lib.omb_server_wait(None, 0, None) == -1
+    assert lib.omb_server_close(None, None) == -1
+
+
 def test_serve_request_bytes(ompblas):
     """Workspace a request holds in the loop: split GEMMs their packed
     operands (rounded to 256 B), plain GEMMs with few tiles and K up to 512

@@ -386,11 +386,18 @@ def test_mtm_split_config_8192_both_distributions(dev, ompblas, dist):
 
 
 @pytest.mark.parametrize("nprod", [6, 3])
-@pytest.mark.parametrize("shape", [(256, 384, 512), (130, 77, 1000), (1, 129, 17)])
+@pytest.mark.parametrize("shape", [(256, 384, 512), (130, 77, 1000), (1, 129, 17),
+                                   (64, 64, 16), (300, 200, 33), (128, 128, 8192),
+                                   (128, 128, 8200)])
 def test_packed_split_path_matches_one_call(dev, ompblas, shape, nprod):
     """omb_split_pack + omb_sgemm_packed (operands split once, reusable) gives
     bitwise the result of omb_sgemm in the same split mode, for both layouts
-    of C."""
+    of C.  With 3 products the two run different loops: the packed operands
+    hold 3 planes (single-buffered fragments), omb_sgemm stages the 2 it
+    reads (fragments double-buffered in registers, slices in pairs), so this
+    pins one loop against the other at 1, 2, 3 and an odd count of 16-k
+    slices, and over K split across workgroups (one 128 x 128 tile with
+    K = 8192 and 8200: 256-k splits, the last one a single slice)."""
     M, N, K = shape
     lib = ompblas.lib()
     a = torch.empty((M, K), device=dev)

@@ -1,5 +1,6 @@
-"""Serving loop (omb_serve_run): mixed GEMM / GEMV / dot requests admitted by
-HBM budget and overlapped on several streams.  Every result must be bitwise
+"""Serving loop (omb_serve_run, and the live server omb_server_*): mixed
+GEMM / GEMV / dot requests admitted by HBM budget and overlapped on several
+streams.  Every result must be bitwise
 the same as the request's solo run through the direct entry points, whatever
 ran beside it; a request whose workspace can never fit is refused."""
 import numpy as np
@@ -324,3 +325,66 @@ def test_serving_config_trace_matches_solo_and_oracle(dev, ompblas):
         # one scalar's error can be small by chance: the larger of 2x the
         # oracle's and a fixed fraction of the summation bound n u sum|xy|
         assert got <= max(2 * ref, 1e-3 * n * 2.0 ** -24 * absdot), (i, got, ref)
+
+
+@pytest.mark.parametrize("mode", [1, 0])
+def test_live_server_matches_solo_bitwise(dev, ompblas, mode):
+    """omb_server_*: requests submitted from two threads at once are admitted
+    by the library's worker as they come (slots, arena and batching as in
+    omb_serve_run); every output is bitwise equal to its solo run, each
+    ticket is published once, and close() reports every request."""
+    import threading
+    trace = small_trace(ompblas, 160, 2.0e5, seed=24)
+    tr = Trace(dev, ompblas, trace, seed=24, mode=mode)
+    reqs = tr.requests()
+    torch.cuda.synchronize()  # operands complete before they are submitted
+    max_bytes = max(ompblas.serve_request_bytes(r) for r in reqs)
+    srv = ompblas.Server(slots=8, max_request_bytes=max_bytes)
+    tickets = [None] * len(reqs)
+
+    def submit(part):
+        for i in part:
+            tickets[i] = srv.submit(reqs[i])
+
+    threads = [threading.Thread(target=submit, args=(range(j, len(reqs), 2),))
+               for j in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert sorted(tickets) == list(range(len(reqs)))
+    results = [srv.wait(t) for t in tickets]
+    assert srv.poll(tickets[0]) == results[0]
+    st = srv.close()
+    torch.cuda.synchronize()
+    assert all(r["status"] == 0 for r in results), results
+    assert all(r["done_s"] >= r["start_s"] >= 0 and r["latency_s"] >= 0 for r in results)
+    mixed = tr.outputs()
+    for i in range(len(reqs)):
+        assert not torch.isnan(mixed[i]).any(), i
+        assert torch.equal(mixed[i], tr.solo(i)), (i, trace[i])
+    want_flop = sum(ompblas.request_flop(k, m, n, kk) for k, m, n, kk, _ in trace)
+    assert st["total_flop"] == pytest.approx(want_flop, rel=1e-12)
+    assert st["max_in_flight"] > 1, st
+    assert st["p50_s"] <= st["p99_s"] <= st["max_s"]
+
+
+def test_live_server_refuses_what_never_fits(dev, ompblas):
+    """A request whose workspace exceeds the server's arena comes back with
+    OMB_EBUSY; the requests around it run and match their solo runs."""
+    trace = [(ompblas.REQ_GEMM, 512, 512, 512, 0.0), (ompblas.REQ_DOT, 0, 4096, 0, 0.0),
+             (ompblas.REQ_GEMV, 300, 700, 0, 0.0)]
+    tr = Trace(dev, ompblas, trace, seed=25, mode=1)
+    reqs = tr.requests()
+    torch.cuda.synchronize()
+    assert ompblas.serve_request_bytes(reqs[0]) > 1 << 20
+    with ompblas.Server(slots=1, max_request_bytes=1 << 20) as srv:
+        tickets = [srv.submit(r) for r in reqs]
+        results = [srv.wait(t) for t in tickets]
+    torch.cuda.synchronize()
+    assert results[0]["status"] == ompblas.EBUSY
+    assert results[1]["status"] == 0 and results[2]["status"] == 0
+    out = tr.outputs()
+    assert torch.isnan(out[0]).all()
+    for i in (1, 2):
+        assert torch.equal(out[i], tr.solo(i)), i
