@@ -782,6 +782,31 @@ def serving_leg(pkg, torch, dev, stream, rate, count, slots):
     out["rate_sweep"] = sweep_at((8000.0, 16000.0, 48000.0, 64000.0, 96000.0), True)
     # the same trace with a stream per request (no grouped launches)
     out["rate_sweep_unbatched"] = sweep_at((32000.0, 64000.0, 96000.0), False)
+
+    def live_at(r):
+        """The same requests through the live server (omb_server_*), submitted
+        from this thread at the trace's arrival times (busy-wait pacing);
+        latency runs from each submission to its observed completion."""
+        tr = pkg.serving_trace(count, r, seed=4)
+        reqs = requests()
+        max_bytes = max(pkg.serve_request_bytes(q) for q in reqs)
+        srv = pkg.Server(slots=slots, max_request_bytes=max_bytes)
+        lags, tickets = [], []
+        t0 = time.perf_counter()
+        for q, t in zip(reqs, tr):
+            while time.perf_counter() - t0 < t[4]:
+                pass
+            lags.append(time.perf_counter() - t0 - t[4])
+            tickets.append(srv.submit(q))
+        done = [srv.wait(t) for t in tickets]
+        s3 = srv.close()
+        return {"rate_per_s": r, "completed": sum(1 for d in done if d["status"] == 0),
+                "p50_ms": s3["p50_s"] * 1e3, "p99_ms": s3["p99_s"] * 1e3,
+                "aggregate_tflops": s3["total_flop"] / s3["wall_s"] / 1e12,
+                "batches": s3["batches"],
+                "submit_lag_p99_ms": float(np.percentile(lags, 99)) * 1e3}
+
+    out["live_submission"] = [live_at(r) for r in (rate, 64000.0)]
     res1, st1 = pkg.serve_run(requests(), slots=1, batching=False)
     out["fifo_one_slot"] = {"p50_ms": st1["p50_s"] * 1e3, "p99_ms": st1["p99_s"] * 1e3,
                             "aggregate_tflops": st1["total_flop"] / st1["wall_s"] / 1e12}

@@ -668,9 +668,10 @@ struct omb_server {
                 continue;
             }
             if (!progress) {
-                if (eng.idle()) {
+                if (eng.idle() && !cfg.spin_poll) {
                     // nothing on the GPU and nothing waiting: sleep until a
-                    // submission (or close) wakes the worker
+                    // submission (or close) wakes the worker (with spin_poll
+                    // it keeps polling, as the trace loop does)
                     std::unique_lock<std::mutex> lock(mu);
                     cv_in.wait_for(lock, std::chrono::milliseconds(10),
                                    [&] { return !inbox.empty() || closing; });
