@@ -297,12 +297,12 @@ omb_status omb_serve_run(omb_request* reqs, int32_t n, const omb_serve_config* c
  * published.  omb_server_poll sets *done (and *result once done);
  * omb_server_wait blocks until the ticket's result is published.
  * omb_server_close waits for every submitted request, frees the server and
- * fills *stats (latencies from submission to observed completion).  With
- * cfg->spin_poll the worker polls without sleeping, also while idle (lowest
- * latency, one host core busy); without it, it sleeps until a submission
- * arrives.  Submit,
- * poll and wait may be called from any threads; close once, after the
- * last submit. */
+ * fills *stats (latencies from submission to observed completion).
+ * With cfg->spin_poll the worker polls without sleeping, also while idle
+ * (lowest latency, one host core busy); without it, it sleeps until a
+ * submission arrives.  Submit, poll and wait may be called from any
+ * threads; close once, after the last submit and after every wait and poll
+ * has returned (it frees the server). */
 typedef struct omb_server omb_server;
 omb_status omb_server_open(const omb_serve_config* cfg, int64_t max_request_bytes,
                            omb_server** server);
