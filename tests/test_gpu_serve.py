@@ -388,3 +388,31 @@ def test_live_server_refuses_what_never_fits(dev, ompblas):
     assert torch.isnan(out[0]).all()
     for i in (1, 2):
         assert torch.equal(out[i], tr.solo(i)), i
+
+
+def test_live_server_wakes_from_sleep(dev, ompblas):
+    """Without spin_poll the idle worker sleeps until a submission arrives:
+    requests submitted 2 ms apart (the worker idle in between) each complete
+    and match their solo runs; polling an unfinished or finished ticket
+    works from the submitting thread."""
+    import time
+    trace = small_trace(ompblas, 12, 1.0e3, seed=26)
+    tr = Trace(dev, ompblas, trace, seed=26, mode=0)
+    reqs = tr.requests()
+    torch.cuda.synchronize()
+    max_bytes = max(ompblas.serve_request_bytes(r) for r in reqs)
+    with ompblas.Server(slots=2, max_request_bytes=max_bytes, spin_poll=False) as srv:
+        results = []
+        for q in reqs:
+            t = srv.submit(q)
+            r = srv.wait(t)
+            assert srv.poll(t) == r
+            results.append(r)
+            time.sleep(0.002)
+        st = srv.close()
+    torch.cuda.synchronize()
+    assert all(r["status"] == 0 for r in results)
+    assert st["max_in_flight"] == 1
+    out = tr.outputs()
+    for i in range(len(reqs)):
+        assert torch.equal(out[i], tr.solo(i)), (i, trace[i])
