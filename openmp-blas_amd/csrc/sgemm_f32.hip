@@ -81,7 +81,9 @@ __device__ inline int clamp4(int64_t r) { return r <= 0 ? 0 : (r >= 4 ? 4 : (int
 // Global -> registers for one BK slice of a BM x BN tile: LA = BM/64 float4
 // of A and LB = BN/64 float4 of B per thread, each along the operand's
 // unit-stride dimension.
-template <int BM, int BN, bool A_KC, bool B_NC, bool VEC>
+// FULL: the slice lies wholly inside A and B and the operands are 16-byte
+// aligned, so every load is an unguarded float4.
+template <int BM, int BN, bool A_KC, bool B_NC, bool VEC, bool FULL = false>
 __device__ inline void load_slice(const Args& p, int64_t m0, int64_t n0, int64_t k0, int tid,
                                   float4 (&ra)[BM / 64], float4 (&rb)[BN / 64]) {
 #pragma unroll
@@ -90,13 +92,21 @@ __device__ inline void load_slice(const Args& p, int64_t m0, int64_t n0, int64_t
         if (A_KC) {  // A[m*lda + k]: float4 along k
             int row = e >> 2, kq = (e & 3) * 4;
             int64_t m = m0 + row, k = k0 + kq;
-            int n = m < p.M ? clamp4(p.K - k) : 0;
-            ra[i] = load4<VEC>(p.A + (n ? m * p.lda + k : 0), n);
+            if constexpr (FULL) {
+                ra[i] = *reinterpret_cast<const float4*>(p.A + m * p.lda + k);
+            } else {
+                int n = m < p.M ? clamp4(p.K - k) : 0;
+                ra[i] = load4<VEC>(p.A + (n ? m * p.lda + k : 0), n);
+            }
         } else {     // A[k*lda + m]: float4 along m
             int mq = (e % (BM / 4)) * 4, kk = e / (BM / 4);
             int64_t m = m0 + mq, k = k0 + kk;
-            int n = k < p.K ? clamp4(p.M - m) : 0;
-            ra[i] = load4<VEC>(p.A + (n ? k * p.lda + m : 0), n);
+            if constexpr (FULL) {
+                ra[i] = *reinterpret_cast<const float4*>(p.A + k * p.lda + m);
+            } else {
+                int n = k < p.K ? clamp4(p.M - m) : 0;
+                ra[i] = load4<VEC>(p.A + (n ? k * p.lda + m : 0), n);
+            }
         }
     }
 #pragma unroll
@@ -105,13 +115,21 @@ __device__ inline void load_slice(const Args& p, int64_t m0, int64_t n0, int64_t
         if (B_NC) {  // B[k*ldb + n]: float4 along n
             int nq = (e % (BN / 4)) * 4, kk = e / (BN / 4);
             int64_t nn = n0 + nq, k = k0 + kk;
-            int n = k < p.K ? clamp4(p.N - nn) : 0;
-            rb[i] = load4<VEC>(p.B + (n ? k * p.ldb + nn : 0), n);
+            if constexpr (FULL) {
+                rb[i] = *reinterpret_cast<const float4*>(p.B + k * p.ldb + nn);
+            } else {
+                int n = k < p.K ? clamp4(p.N - nn) : 0;
+                rb[i] = load4<VEC>(p.B + (n ? k * p.ldb + nn : 0), n);
+            }
         } else {     // B[n*ldb + k]: float4 along k
             int col = e >> 2, kq = (e & 3) * 4;
             int64_t nn = n0 + col, k = k0 + kq;
-            int n = nn < p.N ? clamp4(p.K - k) : 0;
-            rb[i] = load4<VEC>(p.B + (n ? nn * p.ldb + k : 0), n);
+            if constexpr (FULL) {
+                rb[i] = *reinterpret_cast<const float4*>(p.B + nn * p.ldb + k);
+            } else {
+                int n = nn < p.N ? clamp4(p.K - k) : 0;
+                rb[i] = load4<VEC>(p.B + (n ? nn * p.ldb + k : 0), n);
+            }
         }
     }
 }
@@ -236,22 +254,51 @@ __global__ __launch_bounds__(NT, occ_of(BM * BN)) void sgemm_f32_kernel(Args q) 
         }
     };
     float4 ra0[GR], rb0[GC], ra1[GR], rb1[GC];
-    load_slice<BM, BN, A_KC, B_NC, VEC>(p, m0, n0, 0, tid, ra0, rb0);
-    store_slice<BM, BN, A_KC, B_NC>(As[0], Bs[0], tid, ra0, rb0);
-    if (nk > 1) load_slice<BM, BN, A_KC, B_NC, VEC>(p, m0, n0, BK, tid, ra1, rb1);
-    __syncthreads();
-    for (int64_t kt = 0; kt < nk; kt += 2) {
-        if (kt + 2 < nk)
-            load_slice<BM, BN, A_KC, B_NC, VEC>(p, m0, n0, (kt + 2) * BK, tid, ra0, rb0);
-        compute(0, kt);
-        if (kt + 1 < nk) store_slice<BM, BN, A_KC, B_NC>(As[1], Bs[1], tid, ra1, rb1);
+    const bool full = VEC && m0 + BM <= p.M && n0 + BN <= p.N && p.K % BK == 0;
+    if (VEC && full) {
+        // Interior tile: unguarded loads with the slice index clamped to the
+        // last one, whole slice pairs, an odd last slice after the loop.  One
+        // path through the loop keeps the compiler's wait counts exact (a
+        // store waits only for its own loads), so the loads stay two slices
+        // ahead.  Same slices and fold order as the guarded loop below.
+        const int64_t klast = nk - 1;
+        auto kc = [&](int64_t k) { return k < klast ? k : klast; };
+        load_slice<BM, BN, A_KC, B_NC, VEC, true>(p, m0, n0, 0, tid, ra0, rb0);
+        store_slice<BM, BN, A_KC, B_NC>(As[0], Bs[0], tid, ra0, rb0);
+        load_slice<BM, BN, A_KC, B_NC, VEC, true>(p, m0, n0, kc(1) * BK, tid, ra1, rb1);
         __syncthreads();
-        if (kt + 1 >= nk) break;
-        if (kt + 3 < nk)
-            load_slice<BM, BN, A_KC, B_NC, VEC>(p, m0, n0, (kt + 3) * BK, tid, ra1, rb1);
-        compute(1, kt + 1);
-        if (kt + 2 < nk) store_slice<BM, BN, A_KC, B_NC>(As[0], Bs[0], tid, ra0, rb0);
+        int64_t kt = 0;
+        for (; kt + 1 < nk; kt += 2) {
+            load_slice<BM, BN, A_KC, B_NC, VEC, true>(p, m0, n0, kc(kt + 2) * BK, tid, ra0, rb0);
+            __builtin_amdgcn_sched_barrier(0);
+            compute(0, kt);
+            store_slice<BM, BN, A_KC, B_NC>(As[1], Bs[1], tid, ra1, rb1);
+            __syncthreads();
+            load_slice<BM, BN, A_KC, B_NC, VEC, true>(p, m0, n0, kc(kt + 3) * BK, tid, ra1, rb1);
+            __builtin_amdgcn_sched_barrier(0);
+            compute(1, kt + 1);
+            store_slice<BM, BN, A_KC, B_NC>(As[0], Bs[0], tid, ra0, rb0);
+            __syncthreads();
+        }
+        if (kt < nk) compute(0, kt);
+    } else {
+        load_slice<BM, BN, A_KC, B_NC, VEC>(p, m0, n0, 0, tid, ra0, rb0);
+        store_slice<BM, BN, A_KC, B_NC>(As[0], Bs[0], tid, ra0, rb0);
+        if (nk > 1) load_slice<BM, BN, A_KC, B_NC, VEC>(p, m0, n0, BK, tid, ra1, rb1);
         __syncthreads();
+        for (int64_t kt = 0; kt < nk; kt += 2) {
+            if (kt + 2 < nk)
+                load_slice<BM, BN, A_KC, B_NC, VEC>(p, m0, n0, (kt + 2) * BK, tid, ra0, rb0);
+            compute(0, kt);
+            if (kt + 1 < nk) store_slice<BM, BN, A_KC, B_NC>(As[1], Bs[1], tid, ra1, rb1);
+            __syncthreads();
+            if (kt + 1 >= nk) break;
+            if (kt + 3 < nk)
+                load_slice<BM, BN, A_KC, B_NC, VEC>(p, m0, n0, (kt + 3) * BK, tid, ra1, rb1);
+            compute(1, kt + 1);
+            if (kt + 2 < nk) store_slice<BM, BN, A_KC, B_NC>(As[0], Bs[0], tid, ra0, rb0);
+            __syncthreads();
+        }
     }
 
     // Epilogue: rows {g*64 + ty*4 + i}, columns {h*64 + tx*4 + j}.
