@@ -302,7 +302,9 @@ omb_status omb_serve_run(omb_request* reqs, int32_t n, const omb_serve_config* c
  * (lowest latency, one host core busy); without it, it sleeps until a
  * submission arrives.  Submit, poll and wait may be called from any
  * threads; close once, after the last submit and after every wait and poll
- * has returned (it frees the server). */
+ * has returned (it frees the server).  A server keeps each submission's
+ * request and result (about 200 bytes) until it is closed, and takes at
+ * most INT32_MAX submissions: a long-lived caller reopens it. */
 typedef struct omb_server omb_server;
 omb_status omb_server_open(const omb_serve_config* cfg, int64_t max_request_bytes,
                            omb_server** server);
