@@ -254,7 +254,7 @@ __global__ __launch_bounds__(NT, occ_of(BM * BN)) void sgemm_f32_kernel(Args q) 
         }
     };
     float4 ra0[GR], rb0[GC], ra1[GR], rb1[GC];
-    const bool full = VEC && m0 + BM <= p.M && n0 + BN <= p.N && p.K % BK == 0;
+    const bool full = VEC && m0 + BM <= p.M && n0 + BN <= p.N && p.K > 0 && p.K % BK == 0;
     if (VEC && full) {
         // Interior tile: unguarded loads with the slice index clamped to the
         // last one, whole slice pairs, an odd last slice after the loop.  One
